@@ -1,0 +1,96 @@
+/*
+ * sigpde_hip.h -- C ABI of the MI355X (gfx950) signature-kernel PDE solver, libsigpde_hip.so.
+ *
+ * Conventions
+ *   - Every pointer is a device pointer owned by the caller, except where stated.
+ *   - Arrays are C-contiguous unless a stride argument is given. Strides count elements.
+ *   - dtype: SIGPDE_F32 (float storage, the reference's float32 semantics) or SIGPDE_F64.
+ *   - Work is enqueued on `stream` (a hipStream_t; NULL = the default stream) and is
+ *     stream-ordered; the call returns without synchronising.
+ *   - Return value: 0 on success, otherwise a SIGPDE_E* code; sigpde_last_error() then holds a
+ *     message for the calling thread.
+ *   - Refined lengths follow the reference: Lref = ((L - 1) << order) + 1
+ *     (sigpde/utils.py:6-7); with m = Lx - 1 and n = Ly - 1 increments the refined grid of one
+ *     pair has (m << order) x (n << order) cells. Supported: (m << order) <= 4096 rows.
+ *
+ * The reference has no C interface: its "FFI" is the Python/Numba launch layer
+ * sigpde/PDESolvers.py, whose kernel launches these entry points replace one for one.
+ */
+#ifndef SIGPDE_HIP_H
+#define SIGPDE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum sigpde_dtype { SIGPDE_F32 = 0, SIGPDE_F64 = 1 };
+
+enum sigpde_status {
+    SIGPDE_OK = 0,
+    SIGPDE_EINVAL = 1,   /* bad argument (shape, dtype, null pointer, unsupported size) */
+    SIGPDE_EDEVICE = 2,  /* HIP runtime error (launch failure, no device) */
+    SIGPDE_ENOMEM = 3    /* device allocation failed */
+};
+
+/* Message describing the last failure on the calling thread ("" if none). */
+const char *sigpde_last_error(void);
+
+/* Library version string. */
+const char *sigpde_version(void);
+
+/*
+ * Wavefront from precomputed increments, one PDE per pair.
+ * Replaces PairwisePDESolver.solve / solve_scaled (sigpde/PDESolvers.py:56-84), which launch
+ * sigpde_pairwise / sigpde_pairwise_scaled (sigpde/cuda_pairwise_kernels.py:8-85).
+ *   inc : (B, m, n) increments of dtype, pair b at inc + b * inc_stride_b, row stride n
+ *   sx, sy : (B,) scales or NULL; when either is given c = inc * (sy[b] * sx[b]) in dtype
+ *   out : (B,) result k[m<<order][n<<order]
+ */
+int sigpde_pde_pairwise(int dtype, const void *inc, int64_t B, int64_t m, int64_t n,
+                        int64_t inc_stride_b, int order, const void *sx, const void *sy,
+                        void *out, void *stream);
+
+/*
+ * Wavefront from a precomputed increment tile, one PDE per (x, y) pair.
+ * Replaces GramPDESolver.solve / solve_scaled and SymmetricGramPDESolver.solve / solve_scaled
+ * (sigpde/PDESolvers.py:160-190, 209-245), which launch sigpde_gram[_scaled] and
+ * sigpde_gram_symmetric[_scaled] (sigpde/cuda_gram_kernels.py:8-165).
+ *   inc : tile (bx, by, m, n); pair (i, j) at inc + i*stride_x + j*stride_y, row stride n
+ *   sx : (bx,) or NULL, sy : (by,) or NULL; c = inc * (sx[i] * sy[j]) in dtype
+ *   out : result written at out[(off_x + i) * ldo + (off_y + j)]
+ *   symmetric != 0: pairs with off_y + j > off_x + i are skipped and every solved value is
+ *   also written to out[(off_y + j) * ldo + (off_x + i)] (sigpde/cuda_gram_kernels.py:85-86,
+ *   110-113).
+ */
+int sigpde_pde_gram(int dtype, const void *inc, int64_t bx, int64_t by, int64_t m, int64_t n,
+                    int64_t stride_x, int64_t stride_y, int order, const void *sx,
+                    const void *sy, int symmetric, int64_t off_x, int64_t off_y, void *out,
+                    int64_t ldo, void *stream);
+
+/*
+ * Robust normalisation: per-path Chandrupatla root-find of PDE(lambda^2 * Delta) = target.
+ * Replaces RobustSigPDE._norm_factors (sigpde/torch/kernel.py:180-195) with its launches of
+ * sigpde_pairwise and sigpde_pairwise_norm_chandraputla (sigpde/cuda_pairwise_kernels.py:
+ * 8-39, 88-166) through PairwisePDESolver.solve / solve_norms (sigpde/PDESolvers.py:56-101).
+ *   inc : (B, m, m) increments of each path with itself, path b at inc + b * inc_stride_b
+ *   normalizer: SIGPDE_NORM_LOG  -- knorm = K(x,x) is solved here, NaN -> +inf, and the target is
+ *                                  2 - 1/(1 + log knorm) in dtype (sigpde/torch/kernel.py:23-24);
+ *                                  knorm (B,) receives K(x,x);
+ *               SIGPDE_NORM_GIVEN -- knorm (B,) holds K(x,x) and target (B,) the normaliser's
+ *                                  value, both computed by the caller.
+ *   lambda_out : (B,) the root, set to 0 where knorm < 1 (sigpde/torch/kernel.py:195)
+ *   iters_out : (B,) int32 wavefront evaluations of the root-find, or NULL
+ */
+enum sigpde_normalizer { SIGPDE_NORM_LOG = 0, SIGPDE_NORM_GIVEN = 1 };
+
+int sigpde_robust_norm(int dtype, const void *inc, int64_t B, int64_t m, int64_t inc_stride_b,
+                       int order, int normalizer, const void *target, void *knorm, int maxit,
+                       double tol, void *lambda_out, int32_t *iters_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIGPDE_HIP_H */

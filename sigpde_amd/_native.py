@@ -1,0 +1,133 @@
+"""ctypes binding to libsigpde_hip.so (the C ABI declared in include/sigpde_hip.h).
+
+The HIP library is the only compute backend: if it is missing or no GPU is visible, every entry
+point raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsigpde_hip.so")
+
+F32, F64 = 0, 1
+NORM_LOG, NORM_GIVEN = 0, 1
+
+_lib = None
+_lock = threading.Lock()
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_dbl = ctypes.c_double
+
+# name -> argtypes; every entry point returns int status
+_SIGNATURES = {
+    "sigpde_pde_pairwise": [_int, _vp, _i64, _i64, _i64, _i64, _int, _vp, _vp, _vp, _vp],
+    "sigpde_pde_gram": [_int, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _int, _vp, _vp, _int,
+                        _i64, _i64, _vp, _i64, _vp],
+    "sigpde_robust_norm": [_int, _vp, _i64, _i64, _i64, _int, _int, _vp, _vp, _int, _dbl, _vp,
+                           _vp, _vp],
+}
+
+
+class NativeError(RuntimeError):
+    """Raised when a C ABI call returns a non-zero status."""
+
+
+def load(path: str | None = None) -> ctypes.CDLL:
+    """Loads the HIP library (once). Raises if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise ImportError(
+                f"{p} is missing: build it with `python -m sigpde_amd.build` "
+                "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+        lib = ctypes.CDLL(p)
+        for name, argtypes in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = argtypes
+            fn.restype = _int
+        lib.sigpde_last_error.restype = ctypes.c_char_p
+        lib.sigpde_last_error.argtypes = []
+        lib.sigpde_version.restype = ctypes.c_char_p
+        lib.sigpde_version.argtypes = []
+        _lib = lib
+        return lib
+
+
+def version() -> str:
+    return load().sigpde_version().decode()
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().sigpde_last_error().decode()
+        raise NativeError(f"{what} failed (status {rc}): {msg}")
+
+
+def dtype_code(t: torch.dtype) -> int:
+    if t == torch.float32:
+        return F32
+    if t == torch.float64:
+        return F64
+    raise TypeError(f"sigpde_amd supports float32 and float64 paths, got {t}")
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(device: torch.device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _require_gpu(*ts):
+    for t in ts:
+        if t is not None and not t.is_cuda:
+            raise ValueError("sigpde_amd kernels take GPU tensors (device='cuda'); got a CPU tensor")
+
+
+def pde_pairwise(inc: torch.Tensor, order: int, out: torch.Tensor, sx=None, sy=None) -> None:
+    """out[b] = PDE solve of increments inc[b] (B, m, n)."""
+    _require_gpu(inc, out, sx, sy)
+    assert inc.dim() == 3 and inc.stride(2) == 1 and inc.stride(1) == inc.shape[2]
+    B, m, n = inc.shape
+    rc = load().sigpde_pde_pairwise(dtype_code(inc.dtype), _ptr(inc), B, m, n, inc.stride(0),
+                                    int(order), _ptr(sx), _ptr(sy), _ptr(out), _stream(inc.device))
+    _check(rc, "sigpde_pde_pairwise")
+
+
+def pde_gram(inc: torch.Tensor, order: int, out: torch.Tensor, off_x: int = 0, off_y: int = 0,
+             sx=None, sy=None, symmetric: bool = False) -> None:
+    """Solves the (bx, by) pairs of an increment tile into out[off_x + i, off_y + j]."""
+    _require_gpu(inc, out, sx, sy)
+    assert inc.dim() == 4 and inc.stride(3) == 1 and inc.stride(2) == inc.shape[3]
+    assert out.dim() == 2 and out.stride(1) == 1
+    bx, by, m, n = inc.shape
+    rc = load().sigpde_pde_gram(dtype_code(inc.dtype), _ptr(inc), bx, by, m, n, inc.stride(0),
+                                inc.stride(1), int(order), _ptr(sx), _ptr(sy), int(symmetric),
+                                int(off_x), int(off_y), _ptr(out), out.stride(0),
+                                _stream(inc.device))
+    _check(rc, "sigpde_pde_gram")
+
+
+def robust_norm(inc: torch.Tensor, order: int, lam: torch.Tensor, knorm: torch.Tensor,
+                target=None, maxit: int = 100, tol: float = 1e-8, iters=None) -> None:
+    """Chandrupatla normalisation per path. target=None fuses the log normaliser and writes
+    K(x,x) into knorm; otherwise knorm and target are inputs."""
+    _require_gpu(inc, lam, knorm, target, iters)
+    assert inc.dim() == 3 and inc.shape[1] == inc.shape[2] and inc.stride(2) == 1
+    B, m, _ = inc.shape
+    mode = NORM_LOG if target is None else NORM_GIVEN
+    rc = load().sigpde_robust_norm(dtype_code(inc.dtype), _ptr(inc), B, m, inc.stride(0),
+                                   int(order), mode, _ptr(target), _ptr(knorm), int(maxit),
+                                   float(tol), _ptr(lam), _ptr(iters), _stream(inc.device))
+    _check(rc, "sigpde_robust_norm")

@@ -1,0 +1,248 @@
+"""SigPDE / RobustSigPDE: drop-in for stojl/SigPDE's sigpde/torch/kernel.py on MI355X.
+
+Same classes, methods, arguments, defaults and result semantics as the reference
+(sigpde/torch/kernel.py:26-301); the PDE solves and the robust root-find run in the HIP kernels of
+libsigpde_hip.so through sigpde_amd._native.
+
+Semantics kept from the reference:
+  * increments = 4-term second difference of the static kernel's point Gram divided by
+    4**dyadic_order (sigpde/torch/kernel.py:13-21);
+  * y=None means y=x and y_scale=x_scale; a missing scale is ones, a shape-(1,) scale is repeated
+    (kernel.py:32-41, 80-84); gram(x) takes the symmetric path (kernel.py:70-71);
+  * results are new tensors in x's dtype on x's device;
+  * robust: K(x,x) -> NaN to +inf -> normalizer -> Chandrupatla -> lambda = 0 where K(x,x) < 1
+    (kernel.py:180-195); default normalizer 2 - 1/(1 + log x) (kernel.py:23-24);
+    tol=1e-8, maxit=100 defaults.
+  * max_batch bounds the pairs whose increments are materialised at once (pairwise chunks of
+    max_batch, Gram tiles of side ceil(sqrt(max_batch)), kernel.py:55, 68). max_threads is
+    accepted for signature compatibility; the launch geometry is chosen by the library and does
+    not change results.
+Differences (documented, deliberate):
+  * calls are stream-ordered on torch's current stream instead of ending in a device-wide
+    synchronise; torch ops on the result are ordered after the solve.
+  * inputs are validated (3-D, same dtype/device, length >= 2, float32/float64, GPU tensors).
+  * RobustSigPDE.pairwise(x, y) normalises y on y's own length; the reference reuses x's refined
+    length for y's solves (kernel.py:239-247 with PDESolvers.py:43-45), which only agrees when
+    x and y have equal lengths.
+  * symmetric Gram tiles lying entirely above the diagonal are skipped instead of computing and
+    discarding their increments (kernel.py:143-150).
+"""
+from __future__ import annotations
+
+import math
+from typing import Callable, Optional
+
+import torch
+
+from .. import _native
+
+
+def pairwise_inner_product(x, y, static_kernel, dyadic_order):
+    """Increments (B, Lx-1, Ly-1); sigpde/torch/kernel.py:13-16."""
+    ip = static_kernel.pairwise(x, y)
+    d = ip[:, 1:, 1:] + ip[:, :-1, :-1] - ip[:, 1:, :-1] - ip[:, :-1, 1:]
+    return d / float(2 ** (2 * dyadic_order))
+
+
+def gram_inner_product(x, y, static_kernel, dyadic_order):
+    """Increments (Bx, By, Lx-1, Ly-1); sigpde/torch/kernel.py:18-21."""
+    ip = static_kernel.gram(x, y)
+    d = ip[:, :, 1:, 1:] + ip[:, :, :-1, :-1] - ip[:, :, 1:, :-1] - ip[:, :, :-1, 1:]
+    return d / float(2 ** (2 * dyadic_order))
+
+
+def log_normalizer(x):
+    """Default robust normaliser 2 - 1/(1 + log x); sigpde/torch/kernel.py:23-24."""
+    return 2 - 1 / (1 + x.log())
+
+
+def _chunks(total: int, size: int):
+    """(start, stop) chunks of `size`, last one the remainder; sigpde/BatchIterator.py:3-18."""
+    size = max(1, min(size, total))
+    for start in range(0, total, size):
+        yield start, min(start + size, total)
+
+
+def _check_paths(x: torch.Tensor, name: str = "x") -> None:
+    if not isinstance(x, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if x.dim() != 3:
+        raise ValueError(f"{name} must have shape (batch, length, dim), got {tuple(x.shape)}")
+    if x.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f"{name} must be float32 or float64, got {x.dtype}")
+    if not x.is_cuda:
+        raise ValueError(f"{name} must be a GPU tensor; the solver runs only on the device")
+    if x.shape[1] < 2:
+        raise ValueError(f"{name} needs length >= 2, got {x.shape[1]}")
+
+
+def _check_pair(x, y):
+    _check_paths(x, "x")
+    _check_paths(y, "y")
+    if y.dtype != x.dtype or y.device != x.device:
+        raise ValueError("x and y must share dtype and device")
+    if x.shape[2] != y.shape[2]:
+        raise ValueError(f"x and y must have the same dim, got {x.shape[2]} and {y.shape[2]}")
+
+
+def _scale_vector(s, batch: int, like: torch.Tensor) -> torch.Tensor:
+    """Missing -> ones, shape (1,) -> repeated (sigpde/torch/kernel.py:36-41)."""
+    if s is None:
+        return torch.ones(batch, device=like.device, dtype=like.dtype)
+    s = torch.as_tensor(s, device=like.device, dtype=like.dtype).reshape(-1)
+    if s.shape[0] == 1:
+        s = s.repeat(batch)
+    if s.shape[0] != batch:
+        raise ValueError(f"scale has {s.shape[0]} entries for a batch of {batch}")
+    return s.contiguous()
+
+
+class SigPDE:
+    """Untruncated signature kernel by the Goursat PDE (stojl/SigPDE sigpde/torch/kernel.py:26-171)."""
+
+    def __init__(self, static_kernel, dyadic_order: int = 0):
+        if int(dyadic_order) < 0:
+            raise ValueError("dyadic_order must be >= 0")
+        self.static_kernel = static_kernel
+        self.dyadic_order = int(dyadic_order)
+
+    # ------------------------------------------------------------------ pairwise
+    def pairwise(self, x, y=None, x_scale=None, y_scale=None, max_batch=1000, max_threads=1024):
+        y_scale = x_scale if y is None else y_scale
+        y = x if y is None else y
+        _check_pair(x, y)
+        if y.shape[0] != x.shape[0]:
+            raise ValueError("pairwise needs x and y with the same batch size")
+        B = x.shape[0]
+        scaled = x_scale is not None or y_scale is not None
+        sx = _scale_vector(x_scale, B, x) if scaled else None
+        sy = _scale_vector(y_scale, B, y) if scaled else None
+        x, y = x.contiguous(), y.contiguous()
+        out = torch.zeros(B, device=x.device, dtype=x.dtype)
+        for start, stop in _chunks(B, int(max_batch)):
+            inc = pairwise_inner_product(x[start:stop], y[start:stop], self.static_kernel,
+                                         self.dyadic_order).contiguous()
+            _native.pde_pairwise(inc, self.dyadic_order, out[start:stop],
+                                 None if sx is None else sx[start:stop],
+                                 None if sy is None else sy[start:stop])
+        return out
+
+    # ------------------------------------------------------------------ gram
+    def gram(self, x, y=None, x_scale=None, y_scale=None, max_batch=1000, max_threads=1024):
+        tile = math.ceil(math.sqrt(int(max_batch)))
+        if y is None:
+            return self._gram_sym(x, x_scale, tile)
+        return self._gram(x, y, x_scale, y_scale, tile)
+
+    def _gram(self, x, y, x_scale, y_scale, tile):
+        _check_pair(x, y)
+        Bx, By = x.shape[0], y.shape[0]
+        scaled = x_scale is not None or y_scale is not None
+        sx = _scale_vector(x_scale, Bx, x) if scaled else None
+        sy = _scale_vector(y_scale, By, y) if scaled else None
+        x, y = x.contiguous(), y.contiguous()
+        out = torch.zeros((Bx, By), device=x.device, dtype=x.dtype)
+        for ys, ye in _chunks(By, tile):
+            for xs, xe in _chunks(Bx, tile):
+                inc = gram_inner_product(x[xs:xe], y[ys:ye], self.static_kernel,
+                                         self.dyadic_order).contiguous()
+                _native.pde_gram(inc, self.dyadic_order, out, xs, ys,
+                                 None if sx is None else sx[xs:xe],
+                                 None if sy is None else sy[ys:ye])
+        return out
+
+    def _gram_sym(self, x, x_scale, tile):
+        _check_paths(x)
+        B = x.shape[0]
+        s = _scale_vector(x_scale, B, x) if x_scale is not None else None
+        x = x.contiguous()
+        out = torch.zeros((B, B), device=x.device, dtype=x.dtype)
+        for ys, ye in _chunks(B, tile):
+            for xs, xe in _chunks(B, tile):
+                if ys > xe - 1:  # tile entirely above the diagonal
+                    continue
+                inc = gram_inner_product(x[xs:xe], x[ys:ye], self.static_kernel,
+                                         self.dyadic_order).contiguous()
+                _native.pde_gram(inc, self.dyadic_order, out, xs, ys,
+                                 None if s is None else s[xs:xe],
+                                 None if s is None else s[ys:ye], symmetric=True)
+        return out
+
+
+class RobustSigPDE:
+    """Robust (normalised) signature kernel (stojl/SigPDE sigpde/torch/kernel.py:174-301)."""
+
+    def __init__(self, static_kernel, dyadic_order: int = 0, normalizer: Optional[Callable] = None):
+        if int(dyadic_order) < 0:
+            raise ValueError("dyadic_order must be >= 0")
+        self.static_kernel = static_kernel
+        self.dyadic_order = int(dyadic_order)
+        self.normalizer = log_normalizer if normalizer is None else normalizer
+
+    def _norm_factors(self, inc, lam, knorm, normalizer, tol, maxit, iters=None):
+        """sigpde/torch/kernel.py:180-195 on one chunk of self-increments (B, m, m)."""
+        if normalizer is log_normalizer:
+            _native.robust_norm(inc, self.dyadic_order, lam, knorm, None, maxit, tol, iters)
+            return
+        _native.pde_pairwise(inc, self.dyadic_order, knorm)
+        knorm[torch.isnan(knorm)] = torch.inf
+        target = normalizer(knorm)
+        target = torch.as_tensor(target, device=knorm.device, dtype=knorm.dtype).contiguous()
+        _native.robust_norm(inc, self.dyadic_order, lam, knorm, target, maxit, tol, iters)
+
+    def _normalization(self, x, normalizer, tol, maxit, max_batch, iters=None):
+        B = x.shape[0]
+        lam = torch.zeros(B, device=x.device, dtype=x.dtype)
+        knorm = torch.zeros(B, device=x.device, dtype=x.dtype)
+        for start, stop in _chunks(B, int(max_batch)):
+            xi = x[start:stop]
+            inc = pairwise_inner_product(xi, xi, self.static_kernel, self.dyadic_order).contiguous()
+            self._norm_factors(inc, lam[start:stop], knorm[start:stop], normalizer, tol, maxit,
+                               None if iters is None else iters[start:stop])
+        return lam
+
+    def normalization(self, x, normalizer=None, tol=1e-8, maxit=100, max_batch=1000, max_threads=1024):
+        normalizer = self.normalizer if normalizer is None else normalizer
+        _check_paths(x)
+        return self._normalization(x.contiguous(), normalizer, tol, maxit, max_batch)
+
+    def pairwise(self, x, y=None, normalizer=None, tol=1e-8, maxit=100, max_batch=1000, max_threads=1024):
+        symmetric = y is None
+        y = x if symmetric else y
+        normalizer = self.normalizer if normalizer is None else normalizer
+        _check_pair(x, y)
+        if y.shape[0] != x.shape[0]:
+            raise ValueError("pairwise needs x and y with the same batch size")
+        x, y = x.contiguous(), y.contiguous()
+        B = x.shape[0]
+        out = torch.zeros(B, device=x.device, dtype=x.dtype)
+        for start, stop in _chunks(B, int(max_batch)):
+            xi = x[start:stop]
+            x_inc = pairwise_inner_product(xi, xi, self.static_kernel, self.dyadic_order).contiguous()
+            nb = stop - start
+            sx = torch.zeros(nb, device=x.device, dtype=x.dtype)
+            kx = torch.zeros(nb, device=x.device, dtype=x.dtype)
+            self._norm_factors(x_inc, sx, kx, normalizer, tol, maxit)
+            if symmetric:
+                sy, inc = sx, x_inc
+            else:
+                yi = y[start:stop]
+                y_inc = pairwise_inner_product(yi, yi, self.static_kernel, self.dyadic_order).contiguous()
+                sy = torch.zeros(nb, device=x.device, dtype=x.dtype)
+                ky = torch.zeros(nb, device=x.device, dtype=x.dtype)
+                self._norm_factors(y_inc, sy, ky, normalizer, tol, maxit)
+                inc = pairwise_inner_product(xi, yi, self.static_kernel, self.dyadic_order).contiguous()
+            _native.pde_pairwise(inc, self.dyadic_order, out[start:stop], sx, sy)
+        return out
+
+    def gram(self, x, y=None, x_scale=None, y_scale=None, normalizer=None, tol=1e-8, maxit=100,
+             max_batch=1000, max_threads=1024):
+        x_scales = self.normalization(x, normalizer, tol, maxit, max_batch, max_threads) \
+            if x_scale is None else x_scale
+        if y is None:
+            y_scales = None
+        else:
+            y_scales = self.normalization(y, normalizer, tol, maxit, max_batch, max_threads) \
+                if y_scale is None else y_scale
+        return SigPDE(self.static_kernel, self.dyadic_order).gram(x, y, x_scales, y_scales,
+                                                                  max_batch, max_threads)

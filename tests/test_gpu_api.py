@@ -1,0 +1,143 @@
+"""GPU parity of the sigpde.torch drop-in API (SigPDE / RobustSigPDE) against the CPU oracle.
+
+float64: 1e-10 relative. float32: 1e-5 relative on inputs whose point Gram is exact in float32
+(quantized paths), so the comparison does not depend on the GEMM's reduction order; on Brownian
+float32 inputs the reference scheme itself carries reduction-order noise of ~1e-5..5e-5 (the
+oracle's sequential and correctly rounded point Grams differ by that much), so there the test
+bounds the distance to the float64 oracle instead.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import sigpde_amd.torch as sig
+from helpers import brownian_np, max_rel, quantized_paths, rel_fro
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def cuda(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def test_cfg1_pairwise_f64():
+    """SigPDE(LinearKernel(), 1).pairwise(x, y): batch 8, length 32, dim 4, float64."""
+    x = brownian_np(8, 32, 4, 1 / 32, 0)
+    y = brownian_np(8, 32, 4, 1 / 32, 1)
+    got = sig.SigPDE(sig.kernels.LinearKernel(), 1).pairwise(cuda(x), cuda(y)).cpu().numpy()
+    ref = O.sig_pairwise(x, y, 1)
+    assert rel_fro(got, ref) < 1e-10
+    assert max_rel(got, ref) < 1e-10
+
+
+def test_pairwise_y_none_and_scales():
+    x = brownian_np(6, 20, 3, 0.05, 4)
+    k = sig.SigPDE(sig.kernels.LinearKernel(2.0), 0)
+    got = k.pairwise(cuda(x)).cpu().numpy()
+    assert rel_fro(got, O.sig_pairwise(x, x, 0, param=2.0)) < 1e-10
+    s = np.linspace(0.5, 1.0, 6)
+    got = k.pairwise(cuda(x), x_scale=cuda(s)).cpu().numpy()  # y=None -> y_scale = x_scale
+    assert rel_fro(got, O.sig_pairwise(x, x, 0, param=2.0, sx=s, sy=s)) < 1e-10
+    got = k.pairwise(cuda(x), cuda(x), y_scale=cuda(np.array([0.5]))).cpu().numpy()
+    ref = O.sig_pairwise(x, x, 0, param=2.0, sx=np.ones(6), sy=np.full(6, 0.5))
+    assert rel_fro(got, ref) < 1e-10
+
+
+@pytest.mark.parametrize("max_batch", [1000, 10])
+def test_gram_f64_tiles(max_batch):
+    x = brownian_np(13, 16, 3, 0.1, 2)
+    y = brownian_np(11, 21, 3, 0.1, 3)
+    k = sig.SigPDE(sig.kernels.LinearKernel(), 2)
+    got = k.gram(cuda(x), cuda(y), max_batch=max_batch).cpu().numpy()
+    ref = O.sig_gram(x, y, 2)
+    assert rel_fro(got, ref) < 1e-10
+    gs = k.gram(cuda(x), max_batch=max_batch).cpu().numpy()
+    ref_s = O.sig_gram(x, x, 2, symmetric=True)
+    assert rel_fro(gs, ref_s) < 1e-10
+    assert np.array_equal(gs, gs.T)
+
+
+def test_cfg2_gram_f32_quantized():
+    """cfg2 shape (length 100, dim 8, order 2) on a 24 x 20 block, float32 reference semantics."""
+    x = quantized_paths(24, 100, 8, 7)
+    y = quantized_paths(20, 100, 8, 8)
+    got = sig.SigPDE(sig.kernels.LinearKernel(), 2).gram(cuda(x), cuda(y)).cpu().numpy()
+    ref = O.sig_gram(x, y, 2)  # float32 oracle: FP32-ref semantics
+    assert rel_fro(got, ref) < 1e-5
+    assert max_rel(got, ref, floor=1e-2 * np.abs(ref).max()) < 1e-5
+
+
+def test_cfg2_gram_f32_brownian_vs_f64():
+    x = brownian_np(16, 100, 8, 1 / 100, 2, np.float32)
+    y = brownian_np(16, 100, 8, 1 / 100, 3, np.float32)
+    got = sig.SigPDE(sig.kernels.LinearKernel(), 2).gram(cuda(x), cuda(y)).cpu().numpy()
+    ref64 = O.sig_gram(x.astype(np.float64), y.astype(np.float64), 2)
+    ref32 = O.sig_gram(x, y, 2)
+    # same error class as the reference's own float32 semantics
+    assert rel_fro(got, ref64) < 1e-4
+    assert rel_fro(got, ref32) < 1e-4
+
+
+@pytest.mark.parametrize("kind,param", [("rbf", 1.5), ("l1rbf", 2.0)])
+def test_nonlinear_static_kernels(kind, param):
+    x = brownian_np(5, 12, 3, 0.05, 11)
+    y = brownian_np(5, 15, 3, 0.05, 12)
+    K = {"rbf": sig.kernels.RBFKernel, "l1rbf": sig.kernels.L1RBFKernel}[kind](param)
+    k = sig.SigPDE(K, 1)
+    got = k.pairwise(cuda(x), cuda(y[:, :12])).cpu().numpy()
+    assert rel_fro(got, O.sig_pairwise(x, y[:, :12], 1, kind, param)) < 1e-10
+    got = k.gram(cuda(x), cuda(y)).cpu().numpy()
+    assert rel_fro(got, O.sig_gram(x, y, 1, kind, param)) < 1e-10
+
+
+def test_robust_normalization_f64():
+    """cfg3 shape (length 64, dim 16, order 1, N(0, 9/64) increments, tol 1e-10) on 32 paths."""
+    x = brownian_np(32, 64, 16, 9 / 64, 4)
+    rk = sig.RobustSigPDE(sig.kernels.LinearKernel(), 1)
+    lam = rk.normalization(cuda(x), tol=1e-10).cpu().numpy()
+    ref = O.robust_normalization(x, 1, tol=1e-10)
+    assert max_rel(lam, ref, floor=1e-3) < 1e-9
+    assert np.all((lam >= 0) & (lam <= 1))
+
+
+def test_robust_custom_normalizer_matches_oracle():
+    x = brownian_np(10, 30, 4, 0.2, 6)
+
+    def my_norm(v):
+        return 2 - 1 / v.pow(0.1) if isinstance(v, torch.Tensor) else 2 - 1 / np.power(v, 0.1)
+
+    rk = sig.RobustSigPDE(sig.kernels.LinearKernel(), 0)
+    lam = rk.normalization(cuda(x), normalizer=my_norm, tol=1e-10).cpu().numpy()
+    inc = O.second_difference(O.static_pairwise("linear", x, x), 0)
+    ref, _, _ = O.norm_factors(inc, 0, my_norm, tol=1e-10)
+    assert max_rel(lam, ref, floor=1e-3) < 1e-9
+
+
+def test_robust_pairwise_and_gram_f64():
+    x = brownian_np(9, 24, 4, 0.15, 8)
+    y = brownian_np(9, 24, 4, 0.15, 9)
+    rk = sig.RobustSigPDE(sig.kernels.LinearKernel(), 1)
+    lx = O.robust_normalization(x, 1)
+    ly = O.robust_normalization(y, 1)
+    got = rk.pairwise(cuda(x), cuda(y)).cpu().numpy()
+    assert rel_fro(got, O.sig_pairwise(x, y, 1, sx=lx, sy=ly)) < 1e-9
+    got = rk.pairwise(cuda(x)).cpu().numpy()
+    assert rel_fro(got, O.sig_pairwise(x, x, 1, sx=lx, sy=lx)) < 1e-9
+    g = rk.gram(cuda(x)).cpu().numpy()
+    assert rel_fro(g, O.sig_gram(x, x, 1, sx=lx, sy=lx, symmetric=True)) < 1e-9
+    g = rk.gram(cuda(x), cuda(y)).cpu().numpy()
+    assert rel_fro(g, O.sig_gram(x, y, 1, sx=lx, sy=ly)) < 1e-9
+
+
+def test_api_validation():
+    k = sig.SigPDE(sig.kernels.LinearKernel(), 0)
+    with pytest.raises(ValueError):
+        k.pairwise(torch.zeros((2, 1, 3), device=DEV))
+    with pytest.raises(ValueError):
+        k.pairwise(torch.zeros((2, 5, 3)))  # CPU tensor
+    with pytest.raises(TypeError):
+        k.pairwise(torch.zeros((2, 5, 3), device=DEV, dtype=torch.float16))
+    out = k.pairwise(torch.zeros((0, 5, 3), device=DEV))
+    assert out.shape == (0,)
