@@ -1,0 +1,202 @@
+"""Benchmark: signature-kernel Gram pairs/sec on one MI355X.
+
+Workload (default): Gram of X, Y each 1024 Brownian paths, length 256, dim 64, dyadic order 0,
+linear static kernel (increments N(0, 1/256), seeds 5/6, float32 inputs resident in HBM).
+One step = one full 1024 x 1024 Gram. value = pairs/sec = 1024^2 / step time.
+
+Besides the JSON contract fields the line carries:
+  rel_err_fp64   ||K - K64||_F / ||K64||_F on a sampled block, K64 = the float64 oracle on the same
+                 float32 inputs upcast (computed in the CPU-baseline leg, which runs the oracle)
+  latency_ms     p50 / p99 of the per-step times
+  roofline       dominant kernel: achieved vs peak, measured with HIP events on its stream
+  cpu_baseline   the oracle (C restatement, float32 reference semantics) timed on this host's
+                 cores on a sampled block, in pairs/sec
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import sigpde_amd.torch as sig  # noqa: E402
+from sigpde_amd import _native  # noqa: E402
+from sigpde_amd.synthetic import brownian_paths  # noqa: E402
+
+MI355X_HBM_GBS = 8000.0
+MI355X_FP64_VALU_TFLOPS = 78.6
+FLOPS_PER_CELL = 10  # pde_step as written
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--mode", default="fp32", choices=["fp32"])
+    p.add_argument("--paths", type=int, default=1024)
+    p.add_argument("--length", type=int, default=256)
+    p.add_argument("--dim", type=int, default=64)
+    p.add_argument("--order", type=int, default=0)
+    p.add_argument("--max-batch", type=int, default=128 * 128)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline / accuracy leg")
+    p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--sample", type=int, default=32, help="side of the sampled accuracy block")
+    return p.parse_args()
+
+
+def gram_step(args, X, Y):
+    k = sig.SigPDE(sig.kernels.LinearKernel(), args.order)
+    return k.gram(X, Y, max_batch=args.max_batch)
+
+
+def dominant_kernel_roofline(args, X, Y):
+    """Times the wavefront kernel alone on one increment tile with HIP events on torch's current
+    stream (the stream the library launches on)."""
+    tile = int(np.ceil(np.sqrt(args.max_batch)))
+    tx = min(tile, X.shape[0])
+    ty = min(tile, Y.shape[0])
+    from sigpde_amd.torch.kernel import gram_inner_product
+    inc = gram_inner_product(X[:tx], Y[:ty], sig.kernels.LinearKernel(), args.order).contiguous()
+    out = torch.zeros((tx, ty), device=X.device, dtype=X.dtype)
+    _native.pde_gram(inc, args.order, out)
+    torch.cuda.synchronize()
+    reps = 3
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        _native.pde_gram(inc, args.order, out)
+    e1.record(st)
+    torch.cuda.synchronize()
+    dur_s = e0.elapsed_time(e1) / 1e3 / reps
+    m = (X.shape[1] - 1) << args.order
+    n = (Y.shape[1] - 1) << args.order
+    pairs = tx * ty
+    # algorithmic HBM bytes per launch: the increment tile read once + the output written
+    bytes_ = pairs * ((X.shape[1] - 1) * (Y.shape[1] - 1) + 1) * X.element_size()
+    flops = pairs * m * n * FLOPS_PER_CELL
+    return {
+        "kernel": "k_pde_gram (wavefront from HBM increments)",
+        "bound": "hbm",
+        "achieved": bytes_ / dur_s / 1e9,
+        "peak": MI355X_HBM_GBS,
+        "unit": "GB/s",
+        "frac": bytes_ / dur_s / 1e9 / MI355X_HBM_GBS,
+        "traffic": None,
+        "launch_ms": dur_s * 1e3,
+        "pairs_per_launch": pairs,
+        "cells_per_s": pairs * m * n / dur_s,
+        "valu_fp64_tflops": flops / dur_s / 1e12,
+    }
+
+
+def cpu_leg(args, X, Y, K):
+    """Runs the oracle: (1) the float64 reference on a sampled block for the accuracy figure,
+    (2) the float32-semantics restatement timed on this host's cores as the CPU baseline."""
+    import oracle
+
+    s = args.sample
+    Xs = X[:s].double().cpu().numpy()
+    Ys = Y[:s].double().cpu().numpy()
+    K64 = oracle.linear_gram(Xs, Ys, args.order)
+    Kg = K[:s, :s].double().cpu().numpy()
+    rel = float(np.linalg.norm(Kg - K64) / np.linalg.norm(K64))
+    per = np.abs(Kg - K64) / np.abs(K64)
+
+    cores = min(os.cpu_count() or 1, 16)
+    oracle.set_num_threads(cores)
+    X32 = X.cpu().numpy()
+    Y32 = Y.cpu().numpy()
+    side = 8
+    t0 = time.perf_counter()
+    oracle.linear_gram(X32[:side], Y32[:side], args.order)
+    dt = time.perf_counter() - t0
+    # grow the block to about cpu_seconds of work
+    pairs_target = side * side * args.cpu_seconds / max(dt, 1e-6)
+    side2 = int(max(side, min(X.shape[0], np.sqrt(pairs_target))))
+    t0 = time.perf_counter()
+    oracle.linear_gram(X32[:side2], Y32[:side2], args.order)
+    dt2 = time.perf_counter() - t0
+    return {
+        "rel_err_fp64": rel,
+        "rel_err_fp64_max": float(per.max()),
+        "rel_err_fp64_p99": float(np.percentile(per, 99)),
+        "rel_err_sample": f"{s}x{s} pairs",
+    }, {
+        "value": side2 * side2 / dt2,
+        "unit": "pairs/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{side2}x{side2} pairs of the same workload, float32 reference semantics "
+                  f"(C restatement, OpenMP over pairs), {dt2:.1f} s",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is design-only)"}))
+        sys.exit(2)
+    dev = torch.device("cuda:0")
+    X = brownian_paths(args.paths, args.length, args.dim, 1.0 / args.length, 5, torch.float32, dev)
+    Y = brownian_paths(args.paths, args.length, args.dim, 1.0 / args.length, 6, torch.float32, dev)
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        K = gram_step(args, X, Y)
+    torch.cuda.synchronize()
+    step_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        K = gram_step(args, X, Y)
+        torch.cuda.synchronize()
+        step_ms.append((time.perf_counter() - ts) * 1e3)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    ms = total / args.steps * 1e3
+    pairs = X.shape[0] * Y.shape[0]
+
+    roof = dominant_kernel_roofline(args, X, Y)
+    line = {
+        "metric": "Gram pairs/sec (plain FP32 drop-in) at rel err vs FP64 oracle",
+        "value": pairs / (ms / 1e3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"SigPDE(LinearKernel(), {args.order}).gram(X, Y), X,Y "
+                               f"{args.paths} Brownian paths, length {args.length}, dim "
+                               f"{args.dim}, float32", "mode": args.mode,
+                   "paths_x": args.paths, "paths_y": args.paths, "length": args.length,
+                   "dim": args.dim, "dyadic_order": args.order},
+        "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
+                       "p99": float(np.percentile(step_ms, 99))},
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        acc, cpu = cpu_leg(args, X, Y, K)
+        line.update(acc)
+        line["cpu_baseline"] = cpu
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

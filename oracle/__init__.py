@@ -238,29 +238,34 @@ def robust_normalization(X, order, kind="linear", param=1.0, normalizer=None, to
 
 
 # ----------------------------------------------------------------------------- NumPy cross-check
-def pde_pairwise_antidiag(inc: np.ndarray, order: int, scale: Optional[np.ndarray] = None) -> np.ndarray:
-    """Independent FP64 restatement: three live anti-diagonals indexed by row, updated one
+def pde_pairwise_antidiag(inc: np.ndarray, order: int, scale: Optional[np.ndarray] = None,
+                          f32ref: bool = False) -> np.ndarray:
+    """Independent restatement: three live anti-diagonals indexed by row, updated one
     anti-diagonal p = i + j at a time with every row of the diagonal at once, as in the reference's
-    thread-per-row sweep (sigpde/cuda_pairwise_kernels.py:17-39)."""
-    inc = np.asarray(inc, dtype=np.float64)
+    thread-per-row sweep (sigpde/cuda_pairwise_kernels.py:17-39).
+    f32ref=True applies the reference's float32 typing: c = float32(inc * scale), c*c in float32,
+    the rest in float64, every stored cell rounded to float32."""
+    inc = np.asarray(inc, dtype=np.float32 if f32ref else np.float64)
     B, m, n = inc.shape
     M, N = (m << order) + 1, (n << order) + 1  # refined lengths, sigpde/utils.py:6-7
-    out = np.zeros(B)
+    out = np.zeros(B, dtype=np.float32 if f32ref else np.float64)
     for b in range(B):
-        s = 1.0 if scale is None else float(scale[b])
-        d1 = np.ones(M)  # diagonal p-1, indexed by row i (k on the boundary is 1)
-        d2 = np.ones(M)  # diagonal p-2
+        st = np.float32 if f32ref else np.float64
+        s = st(1.0) if scale is None else st(scale[b])
+        d1 = np.ones(M, dtype=st)  # diagonal p-1, indexed by row i (k on the boundary is 1)
+        d2 = np.ones(M, dtype=st)  # diagonal p-2
         for p in range(2, M + N - 1):
             i = np.arange(max(1, p - N + 1), min(M, p))
             j = p - i
-            c = inc[b, (i - 1) >> order, (j - 1) >> order] * s
-            k01 = np.where(i == 1, 1.0, d1[i - 1])
-            k10 = np.where(j == 1, 1.0, d1[i])
-            k00 = np.where((i == 1) | (j == 1), 1.0, d2[i - 1])
-            c2 = c * c
+            c = inc[b, (i - 1) >> order, (j - 1) >> order] * s  # in the storage dtype
+            c2 = (c * c).astype(np.float64)
+            c = c.astype(np.float64)
+            k01 = np.where(i == 1, 1.0, d1[i - 1].astype(np.float64))
+            k10 = np.where(j == 1, 1.0, d1[i].astype(np.float64))
+            k00 = np.where((i == 1) | (j == 1), 1.0, d2[i - 1].astype(np.float64))
             val = (k01 + k10) * ((1.0 + 0.5 * c) + (1.0 / 12) * c2) - k00 * (1.0 - (1.0 / 12) * c2)
-            d0 = np.ones(M)
-            d0[i] = val
+            d0 = np.ones(M, dtype=st)
+            d0[i] = val.astype(st)
             d2, d1 = d1, d0
         out[b] = d1[M - 1]
     return out

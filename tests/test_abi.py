@@ -1,0 +1,62 @@
+"""CPU checks of the C ABI: the library loads, exports every entry point declared in
+include/sigpde_hip.h, and rejects bad arguments before touching the device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from sigpde_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "sigpde_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sigpde_\w+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(_native.LIB_PATH):
+        from sigpde_amd import build
+        build.build()
+    return ctypes.CDLL(_native.LIB_PATH)
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for want in ("sigpde_pde_pairwise", "sigpde_pde_gram", "sigpde_robust_norm",
+                 "sigpde_last_error", "sigpde_version"):
+        assert want in names
+
+
+def test_every_declared_symbol_is_exported(lib):
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_version_and_error_string(lib):
+    L = _native.load()
+    assert "gfx950" in L.sigpde_version().decode()
+    assert L.sigpde_last_error().decode() == ""
+
+
+def test_argument_validation_without_device(lib):
+    L = _native.load()
+    # negative batch, null pointers, oversize grid, unknown dtype: rejected before any HIP call
+    assert L.sigpde_pde_pairwise(0, None, -1, 5, 5, 25, 0, None, None, None, None) == 1
+    assert "bad batch" in L.sigpde_last_error().decode()
+    fake = ctypes.c_void_p(16)
+    assert L.sigpde_pde_pairwise(0, fake, 2, 5, 5, 25, 10, None, None, fake, None) == 1
+    assert "exceeds" in L.sigpde_last_error().decode()
+    assert L.sigpde_pde_pairwise(7, fake, 2, 5, 5, 25, 0, None, None, fake, None) == 1
+    assert "dtype" in L.sigpde_last_error().decode()
+    assert L.sigpde_pde_gram(1, fake, 2, 2, 5, 6, 60, 30, 0, None, None, 1, 0, 0, fake, 2, None) == 1
+    assert "symmetric" in L.sigpde_last_error().decode()
+    assert L.sigpde_robust_norm(1, fake, 2, 5, 25, 0, 9, None, fake, 10, 1e-8, fake, None, None) == 1
+    assert "normalizer" in L.sigpde_last_error().decode()
+    # empty work is a no-op success
+    assert L.sigpde_pde_pairwise(0, None, 0, 5, 5, 25, 0, None, None, None, None) == 0

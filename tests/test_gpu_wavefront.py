@@ -40,7 +40,9 @@ def rand_inc(shape, scale, seed, dtype):
     (3, 64, 3, 0),       # rows exactly one band, few columns (< 64: padded band length)
 ])
 def test_pairwise_from_increments(dtype, B, m, n, order):
-    scale = 0.3 / (1 << order) / max(1.0, np.sqrt(max(m, n) / 30))
+    # refined-cell increments of an (m, n) grid of original ones: c / 4^order with c ~ 1.5/sqrt(mn),
+    # which keeps K of order 1..1e3 at every size (no float32 overflow)
+    scale = 1.5 / np.sqrt(m * n) / 4.0 ** order
     inc = rand_inc((B, m, n), scale, seed=m * 1000 + n + order, dtype=dtype)
     ref = O.pde_pairwise(inc, order)
     t = torch.from_numpy(inc).to(DEV)
