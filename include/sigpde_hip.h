@@ -19,6 +19,7 @@
 #ifndef SIGPDE_HIP_H
 #define SIGPDE_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -88,6 +89,38 @@ enum sigpde_normalizer { SIGPDE_NORM_LOG = 0, SIGPDE_NORM_GIVEN = 1 };
 int sigpde_robust_norm(int dtype, const void *inc, int64_t B, int64_t m, int64_t inc_stride_b,
                        int order, int normalizer, const void *target, void *knorm, int maxit,
                        double tol, void *lambda_out, int32_t *iters_out, void *stream);
+
+/*
+ * Fused linear static kernel (k(x, y) = kscale * <x, y>, sigpde/torch/static_kernels.py:3-11):
+ * increments are formed on the device from the paths and multiplied on MFMA straight into the
+ * wavefront, so no (Bx, By, m, n) increment tensor exists. These replace the chain
+ * LinearKernel.gram / .pairwise -> gram_inner_product / pairwise_inner_product (torch) ->
+ * GramPDESolver / PairwisePDESolver launches (sigpde/torch/kernel.py:13-21, 57-61, 100-122,
+ * 145-167) for the linear kernel.
+ *   precision SIGPDE_PREC_FP16X2: increments split into per-path-scaled fp16 hi + lo, products
+ *     accumulated in fp32 on MFMA; the wavefront runs in double; results in dtype.
+ *   X : (Bx, Lx, d), Y : (By, Ly, d) paths of dtype, C-contiguous; 1 <= d <= 128
+ *   sx (Bx,), sy (By,) : optional scales of dtype (c *= sx[i] * sy[j]), or NULL
+ *   gram: out[i * ldo + j] = K(x_i, y_j); symmetric != 0 requires Y == X and writes the lower
+ *     triangle and its mirror.
+ *   pairwise: out[b] = K(x_b, y_b) for Bx == By == B.
+ *   workspace : device buffer of at least sigpde_linear_workspace_bytes(...) bytes (fp16 hi/lo
+ *     increments and per-path scales); same_paths != 0 when Y is X (one split serves both).
+ */
+enum sigpde_precision { SIGPDE_PREC_FP16X2 = 1 };
+
+size_t sigpde_linear_workspace_bytes(int precision, int64_t Bx, int64_t Lx, int64_t By, int64_t Ly,
+                                     int64_t d, int same_paths);
+
+int sigpde_linear_gram(int dtype, int precision, const void *X, int64_t Bx, int64_t Lx,
+                       const void *Y, int64_t By, int64_t Ly, int64_t d, double kscale, int order,
+                       const void *sx, const void *sy, int symmetric, void *out, int64_t ldo,
+                       void *workspace, size_t workspace_bytes, void *stream);
+
+int sigpde_linear_pairwise(int dtype, int precision, const void *X, const void *Y, int64_t B,
+                           int64_t Lx, int64_t Ly, int64_t d, double kscale, int order,
+                           const void *sx, const void *sy, void *out, void *workspace,
+                           size_t workspace_bytes, void *stream);
 
 #ifdef __cplusplus
 }

@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(_HERE, "libsigpde_hip.so")
 
 F32, F64 = 0, 1
 NORM_LOG, NORM_GIVEN = 0, 1
+PREC_FP16X2 = 1
+PRECISIONS = {"fp16x2": PREC_FP16X2}
 
 _lib = None
 _lock = threading.Lock()
@@ -32,6 +34,10 @@ _SIGNATURES = {
                         _i64, _i64, _vp, _i64, _vp],
     "sigpde_robust_norm": [_int, _vp, _i64, _i64, _i64, _int, _int, _vp, _vp, _int, _dbl, _vp,
                            _vp, _vp],
+    "sigpde_linear_gram": [_int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _dbl, _int, _vp,
+                           _vp, _int, _vp, _i64, _vp, ctypes.c_size_t, _vp],
+    "sigpde_linear_pairwise": [_int, _int, _vp, _vp, _i64, _i64, _i64, _i64, _dbl, _int, _vp,
+                               _vp, _vp, _vp, ctypes.c_size_t, _vp],
 }
 
 
@@ -55,6 +61,8 @@ def load(path: str | None = None) -> ctypes.CDLL:
             fn = getattr(lib, name)
             fn.argtypes = argtypes
             fn.restype = _int
+        lib.sigpde_linear_workspace_bytes.restype = ctypes.c_size_t
+        lib.sigpde_linear_workspace_bytes.argtypes = [_int, _i64, _i64, _i64, _i64, _i64, _int]
         lib.sigpde_last_error.restype = ctypes.c_char_p
         lib.sigpde_last_error.argtypes = []
         lib.sigpde_version.restype = ctypes.c_char_p
@@ -131,3 +139,43 @@ def robust_norm(inc: torch.Tensor, order: int, lam: torch.Tensor, knorm: torch.T
                                    int(order), mode, _ptr(target), _ptr(knorm), int(maxit),
                                    float(tol), _ptr(lam), _ptr(iters), _stream(inc.device))
     _check(rc, "sigpde_robust_norm")
+
+
+def _workspace(nbytes: int, device) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+def linear_gram(X: torch.Tensor, Y: torch.Tensor, order: int, kscale: float, out: torch.Tensor,
+                precision: str = "fp16x2", sx=None, sy=None, symmetric: bool = False) -> None:
+    """Fused linear-kernel Gram: out[i, j] = K(X_i, Y_j), increments on MFMA."""
+    _require_gpu(X, Y, out, sx, sy)
+    assert X.is_contiguous() and Y.is_contiguous() and out.stride(1) == 1
+    Bx, Lx, d = X.shape
+    By, Ly, _ = Y.shape
+    same = X.data_ptr() == Y.data_ptr() and X.shape == Y.shape
+    L = load()
+    prec = PRECISIONS[precision]
+    nbytes = L.sigpde_linear_workspace_bytes(prec, Bx, Lx, By, Ly, d, int(same))
+    ws = _workspace(nbytes, X.device)
+    rc = L.sigpde_linear_gram(dtype_code(X.dtype), prec, _ptr(X), Bx, Lx, _ptr(Y), By, Ly, d,
+                              float(kscale), int(order), _ptr(sx), _ptr(sy), int(symmetric),
+                              _ptr(out), out.stride(0), _ptr(ws), nbytes, _stream(X.device))
+    _check(rc, "sigpde_linear_gram")
+
+
+def linear_pairwise(X: torch.Tensor, Y: torch.Tensor, order: int, kscale: float,
+                    out: torch.Tensor, precision: str = "fp16x2", sx=None, sy=None) -> None:
+    """Fused linear-kernel pairwise: out[b] = K(X_b, Y_b)."""
+    _require_gpu(X, Y, out, sx, sy)
+    assert X.is_contiguous() and Y.is_contiguous()
+    B, Lx, d = X.shape
+    Ly = Y.shape[1]
+    same = X.data_ptr() == Y.data_ptr() and X.shape == Y.shape
+    L = load()
+    prec = PRECISIONS[precision]
+    nbytes = L.sigpde_linear_workspace_bytes(prec, B, Lx, B, Ly, d, int(same))
+    ws = _workspace(nbytes, X.device)
+    rc = L.sigpde_linear_pairwise(dtype_code(X.dtype), prec, _ptr(X), _ptr(Y), B, Lx, Ly, d,
+                                  float(kscale), int(order), _ptr(sx), _ptr(sy), _ptr(out),
+                                  _ptr(ws), nbytes, _stream(X.device))
+    _check(rc, "sigpde_linear_pairwise")

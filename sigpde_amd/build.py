@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsigpde_hip.so")
-SOURCES = ["capi.hip", "wavefront.hip"]
+SOURCES = ["capi.hip", "wavefront.hip", "fused_linear.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -1,0 +1,542 @@
+// Fused linear-kernel signature Gram / pairwise on gfx950: the increment Gram
+//   c[i][j] = kscale * <x_{i+1} - x_i, y_{j+1} - y_j> / 4^order
+// (equal to the reference's 4-term second difference of the point Gram, sigpde/torch/kernel.py:
+// 13-21 with sigpde/torch/static_kernels.py:7-11) is computed on MFMA straight into the wavefront's
+// LDS ring; no increment tensor is ever written to HBM.
+//
+// Precision "fp16x2": each path's increments are formed exactly in double, scaled by a per-path
+// power of two 2^e (so max |dx| * 2^e lies in [2^13, 2^14)) and split into fp16 hi + lo. Products
+// lo*hi + hi*lo + hi*hi (the lo*lo term is below the fp16x2 resolution) accumulate in fp32 on
+// v_mfma_f32_16x16x32_f16, are scaled back in fp32, and the wavefront runs in double
+// (Arith::Mixed). Relative error against the float64 reference: ~1e-7 on Brownian paths.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "capi_util.h"
+#include "ring_solver.h"
+
+namespace sigpde {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+constexpr int kKBlock = 32;  // K of one v_mfma_f32_16x16x32_f16
+
+// ------------------------------------------------------------------ split prepass
+// One block per path: dx = x[s+1] - x[s] in double (exact for float inputs), per-path scale,
+// fp16 hi/lo rows of dpad halves (zero padded), and inv = 2^-e.
+template <typename T>
+__global__ __launch_bounds__(256) void k_split_fp16(const T *__restrict__ X, int64_t B, int L,
+                                                    int d, int dpad, _Float16 *__restrict__ hi,
+                                                    _Float16 *__restrict__ lo,
+                                                    float *__restrict__ inv) {
+  __shared__ double red[256];
+  const int64_t b = blockIdx.x;
+  const T *x = X + b * (int64_t)L * d;
+  const int m = L - 1;
+  double mx = 0.0;
+  for (int i = threadIdx.x; i < m * d; i += blockDim.x) {
+    const int s = i / d, k = i - s * d;
+    double v = (double)x[(s + 1) * d + k] - (double)x[s * d + k];
+    mx = fmax(mx, fabs(v));
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  mx = red[0];
+  const int e = (mx > 0.0 && isfinite(mx)) ? 13 - ilogb(mx) : 0;
+  if (threadIdx.x == 0) inv[b] = ldexpf(1.0f, -e);
+  _Float16 *h = hi + b * (int64_t)m * dpad;
+  _Float16 *l = lo + b * (int64_t)m * dpad;
+  for (int i = threadIdx.x; i < m * dpad; i += blockDim.x) {
+    const int s = i / dpad, k = i - s * dpad;
+    double v = 0.0;
+    if (k < d) v = ldexp((double)x[(s + 1) * d + k] - (double)x[s * d + k], e);
+    const _Float16 vh = (_Float16)v;
+    const _Float16 vl = (_Float16)(v - (double)vh);
+    h[i] = vh;
+    l[i] = vl;
+  }
+}
+
+// ------------------------------------------------------------------ MFMA producer (G = 16)
+// Lane group g (16 lanes) solves pairs (x path xa[g], y path ybase[g] + sp * ystep) of the stream.
+// Chunk k of the stream covers 16 refined columns of one band (16 refined rows) of every group's
+// pair: one 16x16 MFMA tile per group, D[row][col] with A = the band's x-increment rows and
+// B = the chunk's y-increment columns (refined indices repeat original rows / columns 2^order
+// times). Lane l holds D[4(l/16) + e][l % 16] and writes those four ring rows of column l % 16.
+// Pipeline per period p: write chunk p+1's tiles, issue chunk p+2's MFMAs, fetch chunk p+3's B
+// operands (and, at a band boundary, the next band's A operands).
+template <int KB, bool kSharedY>
+struct MfmaProducer {
+  static constexpr int G = 16;
+  static constexpr int NP = 4;           // pairs (lane groups) per wave
+  static constexpr int NB = kSharedY ? 1 : NP;
+  const _Float16 *xh, *xl, *yh, *yl;
+  int dpad;
+  int m, n;                              // original increments per x / y path
+  int64_t xa[NP];                        // x path of each group (-1: idle)
+  int64_t ybase[NP];
+  int ystep;
+  const float *invx, *invy;
+  float kmult;                           // kscale / 4^order
+  const float *sx, *sy;                  // optional extra scales (robust Gram), may be null
+  int64_t ny;                            // valid y paths (ybase + sp*ystep < ny)
+
+  half8 Ah[NP][KB], Al[NP][KB];          // current band's A operands
+  half8 Anh[NP][KB], Anl[NP][KB];        // next band's
+  half8 Bh[NB][KB], Bl[NB][KB];          // chunk being multiplied
+  half8 Bnh[NB][KB], Bnl[NB][KB];        // chunk being fetched
+  float4v acc[NP];
+  float mult[NP];                        // scale of the tiles in acc
+  int accJv0;                            // band column of the tiles in acc
+
+  __device__ __forceinline__ int64_t ypath(int g, int sp) const { return ybase[g] + (int64_t)sp * ystep; }
+
+  __device__ __forceinline__ void fetchA(const PairGeom<G> &geo, int band) {
+    const int lane = threadIdx.x & 63;
+    int ro = (band * G + (lane & 15)) >> geo.order;
+    ro = ro > m - 1 ? m - 1 : ro;
+#pragma unroll
+    for (int g = 0; g < NP; ++g) {
+      const int64_t xp = xa[g] < 0 ? 0 : xa[g];
+      const int64_t off = (xp * m + ro) * dpad + 8 * (lane >> 4);
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        Anh[g][kb] = *reinterpret_cast<const half8 *>(xh + off + kb * kKBlock);
+        Anl[g][kb] = *reinterpret_cast<const half8 *>(xl + off + kb * kKBlock);
+      }
+    }
+  }
+  __device__ __forceinline__ void promoteA() {
+#pragma unroll
+    for (int g = 0; g < NP; ++g)
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        Ah[g][kb] = Anh[g][kb];
+        Al[g][kb] = Anl[g][kb];
+      }
+  }
+  __device__ __forceinline__ void fetchB(const PairGeom<G> &geo, int k) {
+    const int lane = threadIdx.x & 63;
+    int sp, band, jv0;
+    geo.locate(k, sp, band, jv0);
+    const int jv = jv0 + (lane & 15);
+    int co = (jv - 1) >> geo.order;
+    co = co < 0 ? 0 : (co > n - 1 ? n - 1 : co);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      int64_t yp = ypath(q, sp);
+      yp = yp < ny ? yp : ny - 1;
+      const int64_t off = (yp * n + co) * dpad + 8 * (lane >> 4);
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        Bnh[q][kb] = *reinterpret_cast<const half8 *>(yh + off + kb * kKBlock);
+        Bnl[q][kb] = *reinterpret_cast<const half8 *>(yl + off + kb * kKBlock);
+      }
+    }
+  }
+  __device__ __forceinline__ void compute(const PairGeom<G> &geo, int k) {
+    const int lane = threadIdx.x & 63;
+    int sp, band, jv0;
+    geo.locate(k, sp, band, jv0);
+    if (jv0 == 0) promoteA();  // the chunk opens a band: its A operands were fetched ahead
+    // columns outside 1..N (the reset column and the padding) must read as c = 0
+    const int jv = jv0 + (lane & 15);
+    const bool ok = jv >= 1 && jv <= geo.N;
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        const half8 z = {};
+        Bh[q][kb] = ok ? Bnh[q][kb] : z;
+        Bl[q][kb] = ok ? Bnl[q][kb] : z;
+      }
+#pragma unroll
+    for (int g = 0; g < NP; ++g) {
+      const int q = kSharedY ? 0 : g;
+      float4v a = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al[g][kb], Bh[q][kb], a, 0, 0, 0);
+        a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[g][kb], Bl[q][kb], a, 0, 0, 0);
+      }
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb)
+        a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[g][kb], Bh[q][kb], a, 0, 0, 0);
+      acc[g] = a;
+      const int64_t xp = xa[g] < 0 ? 0 : xa[g];
+      int64_t yp = ypath(g, sp);
+      yp = yp < ny ? yp : ny - 1;
+      float mm = kmult * invx[xp] * invy[yp];
+      if (sx) mm *= sx[xp];
+      if (sy) mm *= sy[yp];
+      mult[g] = mm;
+    }
+    accJv0 = jv0;
+  }
+  __device__ __forceinline__ void store(float *ring, const PairGeom<G> &geo, int k) {
+    const int lane = threadIdx.x & 63;
+    const int slot0 = (k * kChunk) % Layout<G>::kRingW;
+    const int c = lane & 15;
+#pragma unroll
+    for (int g = 0; g < NP; ++g) {
+      float *dst = ring + (slot0 + c) * kRingStride + g * G + 4 * (lane >> 4);
+      float2 v0 = make_float2(acc[g][0] * mult[g], acc[g][1] * mult[g]);
+      float2 v1 = make_float2(acc[g][2] * mult[g], acc[g][3] * mult[g]);
+      *reinterpret_cast<float2 *>(dst) = v0;
+      *reinterpret_cast<float2 *>(dst + 2) = v1;
+      if (slot0 == 0 && c < kRingMirror) {
+        float *mir = ring + (Layout<G>::kRingW + c) * kRingStride + g * G + 4 * (lane >> 4);
+        *reinterpret_cast<float2 *>(mir) = v0;
+        *reinterpret_cast<float2 *>(mir + 2) = v1;
+      }
+    }
+  }
+  __device__ __forceinline__ void fetch_ahead(const PairGeom<G> &geo, int k) {
+    fetchB(geo, k);
+    int sp, band, jv0;
+    geo.locate(k, sp, band, jv0);
+    if (jv0 == 0) fetchA(geo, band);
+  }
+
+  __device__ __forceinline__ void prologue(float *ring, const PairGeom<G> &geo) {
+    const int nchunks = geo.columns() / kChunk;
+    fetch_ahead(geo, 0);
+    compute(geo, 0);
+    store(ring, geo, 0);
+    if (nchunks > 1) {
+      fetch_ahead(geo, 1);
+      compute(geo, 1);
+    }
+    if (nchunks > 2) fetch_ahead(geo, 2);
+  }
+  __device__ __forceinline__ void begin_period(float *ring, const PairGeom<G> &geo, int per) {
+    const int nchunks = geo.columns() / kChunk;
+    if (per + 1 < nchunks) store(ring, geo, per + 1);
+    if (per + 2 < nchunks) compute(geo, per + 2);
+    if (per + 3 < nchunks) fetch_ahead(geo, per + 3);
+  }
+};
+
+// ------------------------------------------------------------------ kernels
+// Gram: a wave takes four x paths (a quad) and streams `ys` y paths through them.
+// Symmetric: only y <= x pairs are written, with their mirror.
+template <typename TO>
+struct GramFusedArgs {
+  const _Float16 *xh, *xl, *yh, *yl;
+  const float *invx, *invy;
+  int64_t Bx, By;
+  int m, n, order, dpad, ys;
+  float kmult;
+  int symmetric;
+  TO *out;
+  int64_t ldo;
+};
+
+template <typename TO, int KB>
+__global__ __launch_bounds__(64) void k_linear_gram_mixed(GramFusedArgs<TO> a,
+                                                          const float *sxf, const float *syf) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int G = 16;
+  const int lane = threadIdx.x & 63;
+  const int64_t nq = (a.Bx + 3) / 4;
+  const int64_t nyr = (a.By + a.ys - 1) / a.ys;
+  // XCD-aware order: blocks are dealt round-robin to the 8 XCDs; give consecutive y ranges of the
+  // same x quads to one XCD so its L2 sees the same y operands.
+  const int64_t nblk = nq * nyr;
+  const int64_t bid = blockIdx.x;
+  const int64_t per_xcd = (nblk + 7) / 8;
+  const int64_t lin = (bid % 8) * per_xcd + bid / 8;
+  if (lin >= nblk) return;
+  const int64_t xq = lin % nq, yr = lin / nq;
+  const int64_t y0 = yr * a.ys;
+  int ys = (int)((a.By - y0) < a.ys ? (a.By - y0) : a.ys);
+  if (a.symmetric) {
+    // only y <= max x of the quad
+    const int64_t xmax = xq * 4 + 3 < a.Bx - 1 ? xq * 4 + 3 : a.Bx - 1;
+    if (y0 > xmax) return;
+    if (y0 + ys - 1 > xmax) ys = (int)(xmax - y0 + 1);
+  }
+  PairGeom<G> geo(a.m, a.n, a.order, ys);
+  MfmaProducer<KB, true> P;
+  P.xh = a.xh;
+  P.xl = a.xl;
+  P.yh = a.yh;
+  P.yl = a.yl;
+  P.dpad = a.dpad;
+  P.m = a.m;
+  P.n = a.n;
+  P.ystep = 1;
+  P.invx = a.invx;
+  P.invy = a.invy;
+  P.kmult = a.kmult;
+  P.sx = sxf;
+  P.sy = syf;
+  P.ny = a.By;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int64_t x = xq * 4 + g;
+    P.xa[g] = x < a.Bx ? x : -1;
+    P.ybase[g] = y0;
+  }
+  const int64_t xme = xq * 4 + lane / G;
+  ring_solve<Arith::Mixed, G>(P, geo, 1.0, smem_raw, [&](int sp, double r) {
+    const int64_t y = y0 + sp;
+    if (xme >= a.Bx || y >= a.By) return;
+    if (a.symmetric) {
+      if (y > xme) return;
+      a.out[xme * a.ldo + y] = (TO)r;
+      a.out[y * a.ldo + xme] = (TO)r;
+    } else {
+      a.out[xme * a.ldo + y] = (TO)r;
+    }
+  });
+}
+
+// Pairwise: a wave solves four pairs (x_b, y_b).
+template <typename TO, int KB>
+__global__ __launch_bounds__(64) void k_linear_pairwise_mixed(GramFusedArgs<TO> a,
+                                                              const float *sxf,
+                                                              const float *syf) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int G = 16;
+  const int lane = threadIdx.x & 63;
+  PairGeom<G> geo(a.m, a.n, a.order, 1);
+  MfmaProducer<KB, false> P;
+  P.xh = a.xh;
+  P.xl = a.xl;
+  P.yh = a.yh;
+  P.yl = a.yl;
+  P.dpad = a.dpad;
+  P.m = a.m;
+  P.n = a.n;
+  P.ystep = 0;
+  P.invx = a.invx;
+  P.invy = a.invy;
+  P.kmult = a.kmult;
+  P.sx = sxf;
+  P.sy = syf;
+  P.ny = a.By;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int64_t b = (int64_t)blockIdx.x * 4 + g;
+    P.xa[g] = b < a.Bx ? b : -1;
+    P.ybase[g] = b < a.By ? b : a.By - 1;
+  }
+  const int64_t bme = (int64_t)blockIdx.x * 4 + lane / G;
+  ring_solve<Arith::Mixed, G>(P, geo, 1.0, smem_raw, [&](int, double r) {
+    if (bme < a.Bx) a.out[bme] = (TO)r;
+  });
+}
+
+// float copies of optional scales (the fused producer works in float)
+template <typename T>
+__global__ void k_to_float(const T *src, int64_t n, float *dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+
+// ------------------------------------------------------------------ host side
+struct Workspace {
+  _Float16 *xh, *xl, *yh, *yl;
+  float *invx, *invy, *sxf, *syf;
+};
+
+static inline size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+static int dpad_for(int64_t d) { return (int)((d + kKBlock - 1) / kKBlock * kKBlock); }
+
+static size_t workspace_layout(int64_t Bx, int64_t Lx, int64_t By, int64_t Ly, int64_t d,
+                               int same, char *base, Workspace *w) {
+  const size_t dp = (size_t)dpad_for(d);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char *p = base ? base + off : nullptr;
+    off += align_up(bytes);
+    return p;
+  };
+  const size_t nx = (size_t)Bx * (Lx - 1) * dp, ny = (size_t)By * (Ly - 1) * dp;
+  Workspace t;
+  t.xh = (_Float16 *)take(nx * 2);
+  t.xl = (_Float16 *)take(nx * 2);
+  t.yh = same ? t.xh : (_Float16 *)take(ny * 2);
+  t.yl = same ? t.xl : (_Float16 *)take(ny * 2);
+  t.invx = (float *)take(Bx * 4);
+  t.invy = same ? t.invx : (float *)take(By * 4);
+  t.sxf = (float *)take(Bx * 4);
+  t.syf = (float *)take(By * 4);
+  if (w) *w = t;
+  return off;
+}
+
+template <typename T>
+static hipError_t run_split(const T *X, int64_t B, int64_t L, int64_t d, int dp, _Float16 *h,
+                            _Float16 *l, float *inv, hipStream_t st) {
+  if (B == 0) return hipSuccess;
+  k_split_fp16<T><<<dim3((unsigned)B), dim3(256), 0, st>>>(X, B, (int)L, (int)d, dp, h, l, inv);
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t run_to_float(const void *src, int64_t n, float *dst, hipStream_t st) {
+  if (!src || n == 0) return hipSuccess;
+  k_to_float<T><<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>((const T *)src, n, dst);
+  return hipGetLastError();
+}
+
+template <typename T, int KB>
+static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int64_t Bx, int64_t By,
+                               int64_t Lx, int64_t Ly, int64_t d, double kscale, int order,
+                               const void *sx, const void *sy, int symmetric, void *out,
+                               int64_t ldo, const Workspace &w, hipStream_t st) {
+  const int dp = dpad_for(d);
+  hipError_t e = run_split<T>((const T *)X, Bx, Lx, d, dp, w.xh, w.xl, w.invx, st);
+  if (e != hipSuccess) return e;
+  if (w.yh != w.xh) {
+    e = run_split<T>((const T *)Y, By, Ly, d, dp, w.yh, w.yl, w.invy, st);
+    if (e != hipSuccess) return e;
+  }
+  if ((e = run_to_float<T>(sx, Bx, w.sxf, st)) != hipSuccess) return e;
+  if ((e = run_to_float<T>(sy, By, w.syf, st)) != hipSuccess) return e;
+  GramFusedArgs<T> a{};
+  a.xh = w.xh;
+  a.xl = w.xl;
+  a.yh = w.yh;
+  a.yl = w.yl;
+  a.invx = w.invx;
+  a.invy = w.invy;
+  a.Bx = Bx;
+  a.By = By;
+  a.m = (int)(Lx - 1);
+  a.n = (int)(Ly - 1);
+  a.order = order;
+  a.dpad = dp;
+  a.kmult = (float)(kscale / (double)(1ll << (2 * order)));
+  a.symmetric = symmetric;
+  a.out = (T *)out;
+  a.ldo = ldo;
+  const float *sxf = sx ? w.sxf : nullptr;
+  const float *syf = sy ? w.syf : nullptr;
+  const size_t shmem = wave_lds_bytes<Arith::Mixed, 16>((int)((Ly - 1) << order));
+  if (pairwise) {
+    const unsigned blocks = (unsigned)((Bx + 3) / 4);
+    k_linear_pairwise_mixed<T, KB><<<dim3(blocks), dim3(64), shmem, st>>>(a, sxf, syf);
+  } else {
+    // y paths streamed per wave: enough waves to fill the GPU, long enough streams to amortise
+    // the 15-step pipeline fill of each pair
+    const int64_t nq = (Bx + 3) / 4;
+    int ys = 16;
+    while (ys > 1 && nq * ((By + ys - 1) / ys) < 4096) ys >>= 1;
+    a.ys = ys;
+    const int64_t nblk = nq * ((By + ys - 1) / ys);
+    k_linear_gram_mixed<T, KB><<<dim3((unsigned)nblk), dim3(64), shmem, st>>>(a, sxf, syf);
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t dispatch_kb(bool pairwise, const void *X, const void *Y, int64_t Bx, int64_t By,
+                              int64_t Lx, int64_t Ly, int64_t d, double kscale, int order,
+                              const void *sx, const void *sy, int symmetric, void *out,
+                              int64_t ldo, const Workspace &w, hipStream_t st) {
+  const int kb = dpad_for(d) / kKBlock;
+  switch (kb) {
+    case 1:
+      return launch_fused<T, 1>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
+                                symmetric, out, ldo, w, st);
+    case 2:
+      return launch_fused<T, 2>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
+                                symmetric, out, ldo, w, st);
+    case 3:
+      return launch_fused<T, 3>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
+                                symmetric, out, ldo, w, st);
+    case 4:
+      return launch_fused<T, 4>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
+                                symmetric, out, ldo, w, st);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+constexpr int64_t kMaxDim = 4 * kKBlock;
+
+static int check_fused(int dtype, int precision, const void *X, const void *Y, int64_t Bx,
+                       int64_t By, int64_t Lx, int64_t Ly, int64_t d, int order, void *out,
+                       void *ws) {
+  if (dtype != SIGPDE_F32 && dtype != SIGPDE_F64)
+    return set_error(SIGPDE_EINVAL, "linear: unknown dtype %d", dtype);
+  if (precision != SIGPDE_PREC_FP16X2)
+    return set_error(SIGPDE_EINVAL, "linear: unsupported precision mode %d", precision);
+  if (Bx < 0 || By < 0 || !X || !Y || !out || !ws)
+    return set_error(SIGPDE_EINVAL, "linear: bad batch or null pointer");
+  if (Lx < 2 || Ly < 2) return set_error(SIGPDE_EINVAL, "linear: paths need length >= 2");
+  if (d < 1 || d > kMaxDim) return set_error(SIGPDE_EINVAL, "linear: dim must be in [1, %d]", (int)kMaxDim);
+  if (order < 0 || order > 12) return set_error(SIGPDE_EINVAL, "linear: dyadic order must be in [0, 12]");
+  if (((Lx - 1) << order) > kMaxRefinedRows || ((Ly - 1) << order) > kMaxRefinedRows)
+    return set_error(SIGPDE_EINVAL, "linear: refined grid side (length-1) << order exceeds %d", kMaxRefinedRows);
+  return SIGPDE_OK;
+}
+
+}  // namespace sigpde
+
+using namespace sigpde;
+
+extern "C" size_t sigpde_linear_workspace_bytes(int precision, int64_t Bx, int64_t Lx, int64_t By,
+                                               int64_t Ly, int64_t d, int same_paths) {
+  (void)precision;
+  if (Bx < 0 || By < 0 || Lx < 2 || Ly < 2 || d < 1) return 0;
+  return workspace_layout(Bx, Lx, By, Ly, d, same_paths, nullptr, nullptr);
+}
+
+extern "C" int sigpde_linear_gram(int dtype, int precision, const void *X, int64_t Bx, int64_t Lx,
+                                  const void *Y, int64_t By, int64_t Ly, int64_t d, double kscale,
+                                  int order, const void *sx, const void *sy, int symmetric,
+                                  void *out, int64_t ldo, void *workspace, size_t ws_bytes,
+                                  void *stream) {
+  clear_error();
+  if (int rc = check_fused(dtype, precision, X, Y, Bx, By, Lx, Ly, d, order, out, workspace))
+    return rc;
+  if (Bx == 0 || By == 0) return SIGPDE_OK;
+  if (symmetric && (X != Y || Bx != By || Lx != Ly))
+    return set_error(SIGPDE_EINVAL, "linear_gram: symmetric needs Y == X");
+  const int same = X == Y && Bx == By && Lx == Ly;
+  Workspace w;
+  const size_t need = workspace_layout(Bx, Lx, By, Ly, d, same, (char *)workspace, &w);
+  if (ws_bytes < need)
+    return set_error(SIGPDE_EINVAL, "linear_gram: workspace of %zu bytes, need %zu", ws_bytes, need);
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = dtype == SIGPDE_F32
+                     ? dispatch_kb<float>(false, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
+                                          symmetric, out, ldo, w, st)
+                     : dispatch_kb<double>(false, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
+                                           symmetric, out, ldo, w, st);
+  return hip_status(e, "linear_gram launch");
+}
+
+extern "C" int sigpde_linear_pairwise(int dtype, int precision, const void *X, const void *Y,
+                                      int64_t B, int64_t Lx, int64_t Ly, int64_t d, double kscale,
+                                      int order, const void *sx, const void *sy, void *out,
+                                      void *workspace, size_t ws_bytes, void *stream) {
+  clear_error();
+  if (int rc = check_fused(dtype, precision, X, Y, B, B, Lx, Ly, d, order, out, workspace))
+    return rc;
+  if (B == 0) return SIGPDE_OK;
+  const int same = X == Y && Lx == Ly;
+  Workspace w;
+  const size_t need = workspace_layout(B, Lx, B, Ly, d, same, (char *)workspace, &w);
+  if (ws_bytes < need)
+    return set_error(SIGPDE_EINVAL, "linear_pairwise: workspace of %zu bytes, need %zu", ws_bytes, need);
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = dtype == SIGPDE_F32
+                     ? dispatch_kb<float>(true, X, Y, B, B, Lx, Ly, d, kscale, order, sx, sy, 0,
+                                          out, 1, w, st)
+                     : dispatch_kb<double>(true, X, Y, B, B, Lx, Ly, d, kscale, order, sx, sy, 0,
+                                           out, 1, w, st);
+  return hip_status(e, "linear_pairwise launch");
+}

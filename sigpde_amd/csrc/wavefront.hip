@@ -8,32 +8,9 @@
 #include <string.h>
 
 #include "capi_util.h"
-#include "wave_solver.h"
+#include "ring_solver.h"
 
 namespace sigpde {
-
-// Rows per lane H and band count for a refined grid of M rows: H = ceil(M / (64 * nb)) with at
-// most kMaxH rows per lane, adjusted so that a lane's rows never straddle an original row
-// unevenly (H a multiple of R = 2^order, or a divisor of R).
-constexpr int kMaxH = 16;
-
-static int choose_rows_per_lane(int M, int order) {
-  const int R = 1 << order;
-  int nb = (M + kWave * kMaxH - 1) / (kWave * kMaxH);
-  if (nb < 1) nb = 1;
-  for (;; ++nb) {
-    int H = (M + kWave * nb - 1) / (kWave * nb);
-    if (H < 1) H = 1;
-    if (H >= R) {
-      H = (H + R - 1) / R * R;
-    } else {
-      int p = 1;
-      while (p < H) p <<= 1;
-      H = p;  // power of two below R divides R
-    }
-    if (H <= kMaxH) return H;
-  }
-}
 
 // ------------------------------------------------------------------ pair descriptors
 template <typename T>
@@ -56,47 +33,101 @@ struct GramArgs {
   T *out;
 };
 
-constexpr int kWavesPerBlock = 4;
+template <typename T>
+struct RefArith;
+template <>
+struct RefArith<float> {
+  static constexpr Arith kPlain = Arith::Ref32, kScaled = Arith::Scaled32;
+};
+template <>
+struct RefArith<double> {
+  static constexpr Arith kPlain = Arith::Ref64, kScaled = Arith::Scaled64;
+};
 
-template <typename T, int H>
-__global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pde_pairwise(PairwiseArgs<T> a,
-                                                                        int bndStride) {
-  extern __shared__ unsigned char smem_raw[];
-  const int wid = threadIdx.x / kWave;
-  const int64_t b = (int64_t)blockIdx.x * kWavesPerBlock + wid;
-  if (b >= a.B) return;
-  T *bnd = reinterpret_cast<T *>(smem_raw) + (int64_t)wid * bndStride;
-  T s = (T)1;
-  if (a.sx || a.sy) s = (a.sy ? a.sy[b] : (T)1) * (a.sx ? a.sx[b] : (T)1);
-  T r = wave_solve<T, H, false>(a.inc + b * a.stride_b, a.n, a.m, a.n, a.order, s, bnd);
-  if ((threadIdx.x & (kWave - 1)) == 0) a.out[b] = r;
+// One wave per block; a wave solves Layout<G>::kPairs pairs (one per lane group of G lanes).
+// Increments stream from HBM through the wave's LDS ring.
+template <typename T, int G>
+__global__ __launch_bounds__(kWave) void k_pde_pairwise(PairwiseArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  using L = Layout<G>;
+  const int lane = threadIdx.x & (kWave - 1);
+  PairGeom<G> g(a.m, a.n, a.order);
+  GlobalProducer<T, G> P;
+  P.ld = a.n;
+  P.scaled = a.sx || a.sy;
+#pragma unroll
+  for (int q = 0; q < L::kPairs; ++q) {
+    const int64_t b = (int64_t)blockIdx.x * L::kPairs + q;
+    const bool ok = b < a.B;
+    P.inc[q] = ok ? a.inc + b * a.stride_b : nullptr;
+    P.scale[q] = (P.scaled && ok) ? (a.sy ? a.sy[b] : (T)1) * (a.sx ? a.sx[b] : (T)1) : (T)1;
+  }
+  const int64_t b = (int64_t)blockIdx.x * L::kPairs + lane / G;
+  ring_solve<RefArith<T>::kPlain, G>(P, g, 1.0, smem_raw, [&](int, double r) {
+    if (b < a.B) a.out[b] = (T)r;
+  });
 }
 
-template <typename T, int H>
-__global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pde_gram(GramArgs<T> a, int bndStride) {
-  extern __shared__ unsigned char smem_raw[];
-  const int wid = threadIdx.x / kWave;
-  const int64_t p = (int64_t)blockIdx.x * kWavesPerBlock + wid;
-  if (p >= a.bx * a.by) return;
-  const int64_t i = p / a.by, j = p - (p / a.by) * a.by;
-  const int64_t gi = a.off_x + i, gj = a.off_y + j;
-  if (a.symmetric && gj > gi) return;
-  T *bnd = reinterpret_cast<T *>(smem_raw) + (int64_t)wid * bndStride;
-  T s = (T)1;
-  if (a.sx || a.sy) s = (a.sx ? a.sx[i] : (T)1) * (a.sy ? a.sy[j] : (T)1);
-  T r = wave_solve<T, H, false>(a.inc + i * a.stride_x + j * a.stride_y, a.n, a.m, a.n, a.order,
-                                s, bnd);
-  if ((threadIdx.x & (kWave - 1)) == 0) {
-    a.out[gi * a.ldo + gj] = r;
-    if (a.symmetric && gi != gj) a.out[gj * a.ldo + gi] = r;
+// Pair p of a Gram tile -> (i, j). Symmetric diagonal tiles (off_x == off_y, bx == by) list only
+// the lower triangle j <= i; other tiles list all bx*by pairs (symmetric ones skip j above i).
+__device__ __forceinline__ void tile_pair(int64_t p, int64_t by, bool tri, int64_t &i, int64_t &j) {
+  if (tri) {
+    int64_t r = (int64_t)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+    while (r * (r + 1) / 2 > p) --r;
+    while ((r + 1) * (r + 2) / 2 <= p) ++r;
+    i = r;
+    j = p - r * (r + 1) / 2;
+  } else {
+    i = p / by;
+    j = p - i * by;
   }
 }
 
+template <typename T, int G>
+__global__ __launch_bounds__(kWave) void k_pde_gram(GramArgs<T> a, int64_t npairs, int tri) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  using L = Layout<G>;
+  const int lane = threadIdx.x & (kWave - 1);
+  PairGeom<G> g(a.m, a.n, a.order);
+  GlobalProducer<T, G> P;
+  P.ld = a.n;
+  P.scaled = a.sx || a.sy;
+  int64_t gi[L::kPairs], gj[L::kPairs];
+#pragma unroll
+  for (int q = 0; q < L::kPairs; ++q) {
+    const int64_t p = (int64_t)blockIdx.x * L::kPairs + q;
+    int64_t i = 0, j = 0;
+    bool ok = p < npairs;
+    if (ok) tile_pair(p, a.by, tri, i, j);
+    gi[q] = a.off_x + i;
+    gj[q] = a.off_y + j;
+    if (a.symmetric && gj[q] > gi[q]) ok = false;
+    P.inc[q] = ok ? a.inc + i * a.stride_x + j * a.stride_y : nullptr;
+    P.scale[q] = (P.scaled && ok) ? (a.sx ? a.sx[i] : (T)1) * (a.sy ? a.sy[j] : (T)1) : (T)1;
+    if (!ok) gi[q] = -1;
+  }
+  const int q = lane / G;
+  int64_t xi = gi[0], yj = gj[0];
+#pragma unroll
+  for (int w = 1; w < L::kPairs; ++w)
+    if (q == w) {
+      xi = gi[w];
+      yj = gj[w];
+    }
+  ring_solve<RefArith<T>::kPlain, G>(P, g, 1.0, smem_raw, [&](int, double r) {
+    if (xi >= 0) {
+      a.out[xi * a.ldo + yj] = (T)r;
+      if (a.symmetric && xi != yj) a.out[yj * a.ldo + xi] = (T)r;
+    }
+  });
+}
+
 // ------------------------------------------------------------------ robust normalisation
-// One wave per path. Chandrupatla's bracketing iteration as in
-// sigpde/cuda_pairwise_kernels.py:88-166: every lane carries the same double-precision bracket
-// state; the wavefront result is broadcast with readlane before it is used, so all lanes take the
-// same branches and no shared scratch is written between iterations.
+// Chandrupatla's bracketing iteration as in sigpde/cuda_pairwise_kernels.py:88-166, one path per
+// lane group. Every lane of a group carries the same double-precision bracket state; the wavefront
+// result is broadcast within the group before it is used, so a group's lanes take the same
+// branches, and no shared scratch is written between iterations. The wave iterates until all its
+// groups have stopped; a stopped group keeps its answer.
 template <typename T>
 struct RobustArgs {
   const T *inc;
@@ -120,24 +151,47 @@ __device__ __forceinline__ double log_normalizer(double x) {
   return 2.0 - 1.0 / (1.0 + log(x));
 }
 
-template <typename T, int H>
-__global__ __launch_bounds__(kWave *kWavesPerBlock) void k_robust_norm(RobustArgs<T> a,
-                                                                       int bndStride) {
-  extern __shared__ unsigned char smem_raw[];
-  const int wid = threadIdx.x / kWave;
+// Broadcasts the value held by lane (group base + src) to every lane of its group of G lanes.
+template <int G>
+__device__ __forceinline__ double group_bcast(double v, int src) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t b = (int64_t)blockIdx.x * kWavesPerBlock + wid;
-  if (b >= a.B) return;
-  T *bnd = reinterpret_cast<T *>(smem_raw) + (int64_t)wid * bndStride;
-  const T *inc = a.inc + b * a.stride_b;
+  const int from = ((lane & ~(G - 1)) + src) << 2;
+  int2 a = *reinterpret_cast<int2 *>(&v);
+  int2 r;
+  r.x = __builtin_amdgcn_ds_bpermute(from, a.x);
+  r.y = __builtin_amdgcn_ds_bpermute(from, a.y);
+  return *reinterpret_cast<double *>(&r);
+}
 
-  T K1, Tt;
+template <typename T, int G>
+__global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  using L = Layout<G>;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int q = lane / G;
+  const int64_t b = (int64_t)blockIdx.x * L::kPairs + q;
+  const bool live = b < a.B;
+  PairGeom<G> g(a.m, a.m, a.order);
+  const int rM = (g.M - 1) & (G - 1);
+  GlobalProducer<T, G> P;
+  P.ld = a.m;
+  P.scaled = false;
+#pragma unroll
+  for (int w = 0; w < L::kPairs; ++w) {
+    const int64_t bw = (int64_t)blockIdx.x * L::kPairs + w;
+    P.inc[w] = bw < a.B ? a.inc + bw * a.stride_b : nullptr;
+    P.scale[w] = (T)1;
+  }
+
+  T K1 = (T)1, Tt = (T)1;
   if (a.normalizer == 0) {
     // _norm_factors (sigpde/torch/kernel.py:181-184): K(x,x), NaN -> inf, target
-    K1 = wave_solve<T, H, false>(inc, a.m, a.m, a.m, a.order, (T)1, bnd);
+    double r = 0.0;
+    ring_solve<RefArith<T>::kPlain, G>(P, g, 1.0, smem_raw, [&](int, double v) { r = v; });
+    K1 = (T)group_bcast<G>(r, rM);
     if (isnan(K1)) K1 = (T)INFINITY;
     Tt = log_normalizer<T>(K1);
-  } else {
+  } else if (live) {
     K1 = a.knorm[b];
     Tt = a.target[b];
   }
@@ -148,11 +202,16 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_robust_norm(RobustArg
   double fc = fa, t = 0.5;
   double outv = 0.0;
   int it = 0;
-  while (it < a.maxit) {
+  bool active = live && a.maxit > 0;
+  while (__any(active)) {
     double xt = av + t * (bv - av);
+    double r = 0.0;
+    ring_solve<RefArith<T>::kScaled, G>(P, g, xt * xt, smem_raw, [&](int, double v) { r = v; });
+    double sol = group_bcast<G>(r, rM);
+    if (!active) continue;
     ++it;
-    T sol = wave_solve<T, H, true>(inc, a.m, a.m, a.m, a.order, xt * xt, bnd);
-    double ft = isnan(sol) ? (double)INFINITY : (double)(T)(sol - Tt);
+    T solT = (T)sol;
+    double ft = isnan(solT) ? (double)INFINITY : (double)(T)(solT - Tt);
     if (ft * fa >= 0) {
       cv = av;
       fc = fa;
@@ -164,12 +223,13 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_robust_norm(RobustArg
     }
     av = xt;
     fa = ft;
+    bool stop;
     if (fabs(fa) < fabs(fb)) {
       outv = av;
-      if (fabs(fa) < a.tol) break;
+      stop = fabs(fa) < a.tol;
     } else {
       outv = bv;
-      if (fabs(fb) < a.tol) break;
+      stop = fabs(fb) < a.tol;
     }
     double xi = (av - bv) / (cv - bv);
     double phi = (fa - fb) / (fc - fb);
@@ -178,8 +238,9 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_robust_norm(RobustArg
     else
       t = 0.5;
     if (!(0 < t && t < 1)) t = 0.5;
+    if (stop || it >= a.maxit) active = false;
   }
-  if (lane == 0) {
+  if (live && (lane & (G - 1)) == 0) {
     // scales[norms < 1] = 0 (sigpde/torch/kernel.py:195)
     a.lambda[b] = (K1 < (T)1) ? (T)0 : (T)outv;
     if (a.normalizer == 0) a.knorm[b] = K1;
@@ -187,76 +248,47 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_robust_norm(RobustArg
   }
 }
 
-// ------------------------------------------------------------------ dispatch
-template <template <typename, int> class K, typename T, typename Args>
-struct Dispatch;
+// ------------------------------------------------------------------ launchers
+// Lane groups of 16 (four pairs per wave) need half the LDS per pair and fill the CUs better when
+// there are many pairs; groups of 64 finish a single pair in fewer sequential steps.
+constexpr int64_t kManyPairs = 2048;
 
-#define SIGPDE_H_CASES(X) \
-  X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+template <typename T, int G>
+static size_t lds_for(int N) {
+  return wave_lds_bytes<RefArith<T>::kPlain, G>(N);
+}
 
 template <typename T>
 static hipError_t launch_pairwise(const PairwiseArgs<T> &a, hipStream_t st) {
-  const int M = a.m << a.order, N = a.n << a.order;
-  const int H = choose_rows_per_lane(M, a.order);
-  const bool multiBand = M > kWave * H;
-  const int bndStride = multiBand ? N : 0;
-  const size_t shmem = (size_t)kWavesPerBlock * bndStride * sizeof(T);
-  const dim3 grid((unsigned)((a.B + kWavesPerBlock - 1) / kWavesPerBlock));
-  const dim3 block(kWave * kWavesPerBlock);
-  switch (H) {
-#define X(h)                                                                   \
-  case h:                                                                      \
-    hipLaunchKernelGGL((k_pde_pairwise<T, h>), grid, block, shmem, st, a, bndStride); \
-    break;
-    SIGPDE_H_CASES(X)
-#undef X
-    default:
-      return hipErrorInvalidValue;
+  const int N = a.n << a.order;
+  if (a.B >= kManyPairs) {
+    k_pde_pairwise<T, 16><<<dim3((unsigned)((a.B + 3) / 4)), dim3(kWave), lds_for<T, 16>(N), st>>>(a);
+  } else {
+    k_pde_pairwise<T, 64><<<dim3((unsigned)a.B), dim3(kWave), lds_for<T, 64>(N), st>>>(a);
   }
   return hipGetLastError();
 }
 
 template <typename T>
 static hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st) {
-  const int M = a.m << a.order, N = a.n << a.order;
-  const int H = choose_rows_per_lane(M, a.order);
-  const bool multiBand = M > kWave * H;
-  const int bndStride = multiBand ? N : 0;
-  const size_t shmem = (size_t)kWavesPerBlock * bndStride * sizeof(T);
-  const int64_t np = a.bx * a.by;
-  const dim3 grid((unsigned)((np + kWavesPerBlock - 1) / kWavesPerBlock));
-  const dim3 block(kWave * kWavesPerBlock);
-  switch (H) {
-#define X(h)                                                                 \
-  case h:                                                                    \
-    hipLaunchKernelGGL((k_pde_gram<T, h>), grid, block, shmem, st, a, bndStride); \
-    break;
-    SIGPDE_H_CASES(X)
-#undef X
-    default:
-      return hipErrorInvalidValue;
+  const int N = a.n << a.order;
+  const bool tri = a.symmetric && a.off_x == a.off_y && a.bx == a.by;
+  const int64_t np = tri ? a.bx * (a.bx + 1) / 2 : a.bx * a.by;
+  if (np >= kManyPairs) {
+    k_pde_gram<T, 16><<<dim3((unsigned)((np + 3) / 4)), dim3(kWave), lds_for<T, 16>(N), st>>>(a, np, (int)tri);
+  } else {
+    k_pde_gram<T, 64><<<dim3((unsigned)np), dim3(kWave), lds_for<T, 64>(N), st>>>(a, np, (int)tri);
   }
   return hipGetLastError();
 }
 
 template <typename T>
 static hipError_t launch_robust(const RobustArgs<T> &a, hipStream_t st) {
-  const int M = a.m << a.order, N = M;
-  const int H = choose_rows_per_lane(M, a.order);
-  const bool multiBand = M > kWave * H;
-  const int bndStride = multiBand ? N : 0;
-  const size_t shmem = (size_t)kWavesPerBlock * bndStride * sizeof(T);
-  const dim3 grid((unsigned)((a.B + kWavesPerBlock - 1) / kWavesPerBlock));
-  const dim3 block(kWave * kWavesPerBlock);
-  switch (H) {
-#define X(h)                                                                    \
-  case h:                                                                       \
-    hipLaunchKernelGGL((k_robust_norm<T, h>), grid, block, shmem, st, a, bndStride); \
-    break;
-    SIGPDE_H_CASES(X)
-#undef X
-    default:
-      return hipErrorInvalidValue;
+  const int N = a.m << a.order;
+  if (a.B >= kManyPairs) {
+    k_robust_norm<T, 16><<<dim3((unsigned)((a.B + 3) / 4)), dim3(kWave), lds_for<T, 16>(N), st>>>(a);
+  } else {
+    k_robust_norm<T, 64><<<dim3((unsigned)a.B), dim3(kWave), lds_for<T, 64>(N), st>>>(a);
   }
   return hipGetLastError();
 }

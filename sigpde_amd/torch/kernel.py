@@ -97,17 +97,45 @@ def _scale_vector(s, batch: int, like: torch.Tensor) -> torch.Tensor:
     return s.contiguous()
 
 
-class SigPDE:
-    """Untruncated signature kernel by the Goursat PDE (stojl/SigPDE sigpde/torch/kernel.py:26-171)."""
+PRECISIONS = ("ref", "fp16x2")
 
-    def __init__(self, static_kernel, dyadic_order: int = 0):
+
+def _fused_scale(static_kernel, precision: str):
+    """kscale when the call takes the fused linear-kernel kernels, else None."""
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {PRECISIONS}, got {precision!r}")
+    if precision == "ref":
+        return None
+    scale = getattr(static_kernel, "fused_linear_scale", None)
+    if scale is None:
+        raise ValueError(f"precision={precision!r} needs the linear static kernel "
+                         "(kernels.LinearKernel); other static kernels run with precision='ref'")
+    return float(scale)
+
+
+class SigPDE:
+    """Untruncated signature kernel by the Goursat PDE (stojl/SigPDE sigpde/torch/kernel.py:26-171).
+
+    precision (an addition to the reference's signature; per instance, overridable per call):
+      "ref"    -- the reference's arithmetic: increments from the static kernel's point Gram
+                  (4-term second difference, in the inputs' dtype), float32 storage with double
+                  per-cell arithmetic for float32 inputs, all-double for float64 inputs;
+      "fp16x2" -- linear kernel only: increments <dx, dy> formed from the paths and multiplied as
+                  per-path-scaled fp16 hi/lo pairs on MFMA (fp32 accumulate), double wavefront;
+                  relative error ~1e-7 against the float64 reference scheme.
+    """
+
+    def __init__(self, static_kernel, dyadic_order: int = 0, precision: str = "ref"):
         if int(dyadic_order) < 0:
             raise ValueError("dyadic_order must be >= 0")
+        _fused_scale(static_kernel, precision)
         self.static_kernel = static_kernel
         self.dyadic_order = int(dyadic_order)
+        self.precision = precision
 
     # ------------------------------------------------------------------ pairwise
-    def pairwise(self, x, y=None, x_scale=None, y_scale=None, max_batch=1000, max_threads=1024):
+    def pairwise(self, x, y=None, x_scale=None, y_scale=None, max_batch=1000, max_threads=1024,
+                 precision=None):
         y_scale = x_scale if y is None else y_scale
         y = x if y is None else y
         _check_pair(x, y)
@@ -119,6 +147,11 @@ class SigPDE:
         sy = _scale_vector(y_scale, B, y) if scaled else None
         x, y = x.contiguous(), y.contiguous()
         out = torch.zeros(B, device=x.device, dtype=x.dtype)
+        kscale = _fused_scale(self.static_kernel, precision or self.precision)
+        if kscale is not None:
+            _native.linear_pairwise(x, y, self.dyadic_order, kscale, out, precision or self.precision,
+                                    sx, sy)
+            return out
         for start, stop in _chunks(B, int(max_batch)):
             inc = pairwise_inner_product(x[start:stop], y[start:stop], self.static_kernel,
                                          self.dyadic_order).contiguous()
@@ -128,11 +161,31 @@ class SigPDE:
         return out
 
     # ------------------------------------------------------------------ gram
-    def gram(self, x, y=None, x_scale=None, y_scale=None, max_batch=1000, max_threads=1024):
+    def gram(self, x, y=None, x_scale=None, y_scale=None, max_batch=1000, max_threads=1024,
+             precision=None):
+        precision = precision or self.precision
+        kscale = _fused_scale(self.static_kernel, precision)
+        if kscale is not None:
+            return self._gram_fused(x, y, x_scale, y_scale, kscale, precision)
         tile = math.ceil(math.sqrt(int(max_batch)))
         if y is None:
             return self._gram_sym(x, x_scale, tile)
         return self._gram(x, y, x_scale, y_scale, tile)
+
+    def _gram_fused(self, x, y, x_scale, y_scale, kscale, precision):
+        symmetric = y is None
+        if symmetric:
+            y, y_scale = x, x_scale
+        _check_pair(x, y)
+        Bx, By = x.shape[0], y.shape[0]
+        scaled = x_scale is not None or y_scale is not None
+        sx = _scale_vector(x_scale, Bx, x) if scaled else None
+        sy = (sx if symmetric else _scale_vector(y_scale, By, y)) if scaled else None
+        x = x.contiguous()
+        y = x if symmetric else y.contiguous()
+        out = torch.zeros((Bx, By), device=x.device, dtype=x.dtype)
+        _native.linear_gram(x, y, self.dyadic_order, kscale, out, precision, sx, sy, symmetric)
+        return out
 
     def _gram(self, x, y, x_scale, y_scale, tile):
         _check_pair(x, y)
