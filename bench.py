@@ -2,7 +2,9 @@
 
 Workload (default): Gram of X, Y each 1024 Brownian paths, length 256, dim 64, dyadic order 0,
 linear static kernel (increments N(0, 1/256), seeds 5/6, float32 inputs resident in HBM).
-One step = one full 1024 x 1024 Gram. value = pairs/sec = 1024^2 / step time.
+One step = one full 1024 x 1024 Gram. value = pairs/sec = 1024^2 / step time of the
+mixed-precision path (precision="fp16x2": fp16 hi/lo increment Gram on MFMA, double wavefront);
+the plain FP32 drop-in (precision="ref", the reference's arithmetic) is timed beside it.
 
 Besides the JSON contract fields the line carries:
   rel_err_fp64   ||K - K64||_F / ||K64||_F on a sampled block, K64 = the float64 oracle on the same
@@ -33,6 +35,7 @@ from sigpde_amd.synthetic import brownian_paths  # noqa: E402
 
 MI355X_HBM_GBS = 8000.0
 MI355X_FP64_VALU_TFLOPS = 78.6
+MI355X_FP16_MFMA_TFLOPS = 2500.0
 FLOPS_PER_CELL = 10  # pde_step as written
 
 
@@ -41,7 +44,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--mode", default="fp32", choices=["fp32"])
+    p.add_argument("--mode", default="fp16x2", choices=["fp16x2", "ref"])
+    p.add_argument("--ref-steps", type=int, default=1, help="timed steps of the plain FP32 drop-in")
     p.add_argument("--paths", type=int, default=1024)
     p.add_argument("--length", type=int, default=256)
     p.add_argument("--dim", type=int, default=64)
@@ -53,9 +57,62 @@ def parse():
     return p.parse_args()
 
 
-def gram_step(args, X, Y):
-    k = sig.SigPDE(sig.kernels.LinearKernel(), args.order)
+def gram_step(args, X, Y, mode=None):
+    k = sig.SigPDE(sig.kernels.LinearKernel(), args.order, precision=mode or args.mode)
     return k.gram(X, Y, max_batch=args.max_batch)
+
+
+def timed(args, X, Y, mode, steps, warmup):
+    for _ in range(warmup):
+        K = gram_step(args, X, Y, mode)
+    torch.cuda.synchronize()
+    step_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        K = gram_step(args, X, Y, mode)
+        torch.cuda.synchronize()
+        step_ms.append((time.perf_counter() - ts) * 1e3)
+    total = time.perf_counter() - t0
+    return K, total / steps * 1e3, step_ms
+
+
+def fused_roofline(args, X, Y, ms):
+    """The fused kernel is one launch per step (plus a small split prepass); its time is the step
+    time measured with HIP events on torch's current stream, the stream it is launched on."""
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    reps = 2
+    gram_step(args, X, Y, "fp16x2")
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(reps):
+        gram_step(args, X, Y, "fp16x2")
+    e1.record(st)
+    torch.cuda.synchronize()
+    dur_s = e0.elapsed_time(e1) / 1e3 / reps
+    m = (X.shape[1] - 1) << args.order
+    n = (Y.shape[1] - 1) << args.order
+    pairs = X.shape[0] * Y.shape[0]
+    cells = pairs * m * n
+    flops_wave = cells * FLOPS_PER_CELL
+    flops_inc = pairs * 2 * args.dim * (X.shape[1] - 1) * (Y.shape[1] - 1)
+    return {
+        "kernel": "k_linear_gram_mixed (fp16x2 MFMA increments + double wavefront, one launch)",
+        "bound": "valu",
+        "achieved": flops_wave / dur_s / 1e12,
+        "peak": MI355X_FP64_VALU_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": flops_wave / dur_s / 1e12 / MI355X_FP64_VALU_TFLOPS,
+        "traffic": None,
+        "launch_ms": dur_s * 1e3,
+        "cells_per_s": cells / dur_s,
+        "increment_gram_tflops": flops_inc / dur_s / 1e12,
+        "increment_gram_mfma_frac": flops_inc / dur_s / 1e12 / MI355X_FP16_MFMA_TFLOPS,
+        "note": "wavefront flops = 10 per refined cell (pde_step as written); the increment Gram "
+                "(2*d flops per increment) runs on MFMA beside it",
+    }
 
 
 def dominant_kernel_roofline(args, X, Y):
@@ -100,8 +157,8 @@ def dominant_kernel_roofline(args, X, Y):
     }
 
 
-def cpu_leg(args, X, Y, K):
-    """Runs the oracle: (1) the float64 reference on a sampled block for the accuracy figure,
+def cpu_leg(args, X, Y, K, Kref):
+    """Runs the oracle: (1) the float64 reference on a sampled block for the accuracy figures,
     (2) the float32-semantics restatement timed on this host's cores as the CPU baseline."""
     import oracle
 
@@ -112,6 +169,8 @@ def cpu_leg(args, X, Y, K):
     Kg = K[:s, :s].double().cpu().numpy()
     rel = float(np.linalg.norm(Kg - K64) / np.linalg.norm(K64))
     per = np.abs(Kg - K64) / np.abs(K64)
+    Kr = Kref[:s, :s].double().cpu().numpy()
+    rel_ref = float(np.linalg.norm(Kr - K64) / np.linalg.norm(K64))
 
     cores = min(os.cpu_count() or 1, 16)
     oracle.set_num_threads(cores)
@@ -132,6 +191,7 @@ def cpu_leg(args, X, Y, K):
         "rel_err_fp64_max": float(per.max()),
         "rel_err_fp64_p99": float(np.percentile(per, 99)),
         "rel_err_sample": f"{s}x{s} pairs",
+        "plain_fp32_rel_err_fp64": rel_ref,
     }, {
         "value": side2 * side2 / dt2,
         "unit": "pairs/s",
@@ -153,24 +213,15 @@ def main():
     Y = brownian_paths(args.paths, args.length, args.dim, 1.0 / args.length, 6, torch.float32, dev)
     torch.cuda.synchronize()
 
-    for _ in range(args.warmup):
-        K = gram_step(args, X, Y)
-    torch.cuda.synchronize()
-    step_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        K = gram_step(args, X, Y)
-        torch.cuda.synchronize()
-        step_ms.append((time.perf_counter() - ts) * 1e3)
-    torch.cuda.synchronize()
-    total = time.perf_counter() - t0
-    ms = total / args.steps * 1e3
+    K, ms, step_ms = timed(args, X, Y, args.mode, args.steps, args.warmup)
     pairs = X.shape[0] * Y.shape[0]
+    # the plain FP32 drop-in, same workload, for the "vs plain FP32" comparison
+    Kref, ms_ref, _ = timed(args, X, Y, "ref", args.ref_steps, 0) if args.mode != "ref" else (K, ms, step_ms)
 
-    roof = dominant_kernel_roofline(args, X, Y)
+    roof = fused_roofline(args, X, Y, ms) if args.mode == "fp16x2" else dominant_kernel_roofline(args, X, Y)
     line = {
-        "metric": "Gram pairs/sec (plain FP32 drop-in) at rel err vs FP64 oracle",
+        "metric": "Gram pairs/sec (mixed-precision fp16x2 path; plain FP32 drop-in beside it) at "
+                  "rel err vs FP64 oracle",
         "value": pairs / (ms / 1e3),
         "unit": "pairs/s",
         "n_gpus": 1,
@@ -180,19 +231,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "fp16x2+f64" if args.mode == "fp16x2" else "f32",
         "data": "synthetic",
         "config": {"workload": f"SigPDE(LinearKernel(), {args.order}).gram(X, Y), X,Y "
                                f"{args.paths} Brownian paths, length {args.length}, dim "
-                               f"{args.dim}, float32", "mode": args.mode,
+                               f"{args.dim}, float32 inputs", "mode": args.mode,
                    "paths_x": args.paths, "paths_y": args.paths, "length": args.length,
                    "dim": args.dim, "dyadic_order": args.order},
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
                        "p99": float(np.percentile(step_ms, 99))},
+        "plain_fp32": {"value": pairs / (ms_ref / 1e3), "unit": "pairs/s", "ms_per_step": ms_ref},
+        "speedup_vs_plain_fp32": ms_ref / ms,
         "roofline": roof,
     }
     if not args.no_cpu:
-        acc, cpu = cpu_leg(args, X, Y, K)
+        acc, cpu = cpu_leg(args, X, Y, K, Kref)
         line.update(acc)
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)

@@ -250,8 +250,8 @@ __global__ __launch_bounds__(64) void k_linear_gram_mixed(GramFusedArgs<TO> a,
   // XCD-aware order: blocks are dealt round-robin to the 8 XCDs; give consecutive y ranges of the
   // same x quads to one XCD so its L2 sees the same y operands.
   const int64_t nblk = nq * nyr;
-  const int64_t bid = blockIdx.x;
-  const int64_t per_xcd = (nblk + 7) / 8;
+  const int64_t bid = blockIdx.x;             // grid = 8 * per_xcd blocks
+  const int64_t per_xcd = gridDim.x / 8;
   const int64_t lin = (bid % 8) * per_xcd + bid / 8;
   if (lin >= nblk) return;
   const int64_t xq = lin % nq, yr = lin / nq;
@@ -435,7 +435,9 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
     while (ys > 1 && nq * ((By + ys - 1) / ys) < 4096) ys >>= 1;
     a.ys = ys;
     const int64_t nblk = nq * ((By + ys - 1) / ys);
-    k_linear_gram_mixed<T, KB><<<dim3((unsigned)nblk), dim3(64), shmem, st>>>(a, sxf, syf);
+    // a multiple of 8 blocks so that the XCD-major block order is a bijection onto the work units
+    const int64_t grid = (nblk + 7) / 8 * 8;
+    k_linear_gram_mixed<T, KB><<<dim3((unsigned)grid), dim3(64), shmem, st>>>(a, sxf, syf);
   }
   return hipGetLastError();
 }

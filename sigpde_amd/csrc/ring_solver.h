@@ -339,7 +339,7 @@ __device__ void ring_solve(Producer &P, const PairGeom<G> &g, double s, void *ld
     const int t0 = per * kChunk;
     const bool top = band0 == 0;
     double result = 0.0;
-    if (tM >= t0 && tM < t0 + kChunk) {
+    if (sp < g.stream && tM >= t0 && tM < t0 + kChunk) {
       consume_period<A, G, true, true>(ring, bnd, g, s, t0, rst0, jw0, top, slot, k, kPrev, tM,
                                        rM, result);
       if (rho == rM) out(sp, result);
