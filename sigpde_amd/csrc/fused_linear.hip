@@ -69,9 +69,12 @@ __global__ __launch_bounds__(256) void k_split_fp16(const T *__restrict__ X, int
 // Chunk k of the stream covers 16 refined columns of one band (16 refined rows) of every group's
 // pair: one 16x16 MFMA tile per group, D[row][col] with A = the band's x-increment rows and
 // B = the chunk's y-increment columns (refined indices repeat original rows / columns 2^order
-// times). Lane l holds D[4(l/16) + e][l % 16] and writes those four ring rows of column l % 16.
-// Pipeline per period p: write chunk p+1's tiles, issue chunk p+2's MFMAs, fetch chunk p+3's B
-// operands (and, at a band boundary, the next band's A operands).
+// times; columns outside 1..N read a clamped column, see ring_solver.h). Lane l holds
+// D[4(l/16) + e][l % 16] and writes those four ring rows of column l % 16.
+// Pipeline, run before the last consumer group of period p: write chunk p+1's tiles, issue chunk
+// p+2's MFMAs, fetch chunk p+3's B operands; after the last chunk of a band is multiplied, the
+// next band's A operands are loaded. Operand rows are addressed as a wave-uniform 64-bit row base
+// plus a 32-bit per-lane offset.
 template <int KB, bool kSharedY>
 struct MfmaProducer {
   static constexpr int G = 16;
@@ -88,75 +91,72 @@ struct MfmaProducer {
   const float *sx, *sy;                  // optional extra scales (robust Gram), may be null
   int64_t ny;                            // valid y paths (ybase + sp*ystep < ny)
 
-  half8 Ah[NP][KB], Al[NP][KB];          // current band's A operands
-  half8 Anh[NP][KB], Anl[NP][KB];        // next band's
-  half8 Bh[NB][KB], Bl[NB][KB];          // chunk being multiplied
-  half8 Bnh[NB][KB], Bnl[NB][KB];        // chunk being fetched
+  half8 Ah[NP][KB], Al[NP][KB];          // A operands of the band being multiplied
+  half8 Bh[NB][KB], Bl[NB][KB];          // next chunk's B operands (fetched one period ahead)
   float4v acc[NP];
   float mult[NP];                        // scale of the tiles in acc
-  int accJv0;                            // band column of the tiles in acc
+  float pm[NP];                          // scale of the pair being multiplied (per group)
+  int64_t yrow[NB];                      // first B row (in halves) of the pair being fetched
+  Cursor<G> cB, cC;                      // next chunk to fetch B for / to multiply
+  int slotS;                             // ring slot of the next chunk to store
 
-  __device__ __forceinline__ int64_t ypath(int g, int sp) const { return ybase[g] + (int64_t)sp * ystep; }
+  __device__ __forceinline__ int64_t ypath(int g, int sp) const {
+    const int64_t yp = ybase[g] + (int64_t)sp * ystep;
+    return yp < ny ? yp : ny - 1;
+  }
 
   __device__ __forceinline__ void fetchA(const PairGeom<G> &geo, int band) {
     const int lane = threadIdx.x & 63;
     int ro = (band * G + (lane & 15)) >> geo.order;
     ro = ro > m - 1 ? m - 1 : ro;
+    const uint32_t lo = (uint32_t)(ro * dpad + 8 * (lane >> 4));
 #pragma unroll
     for (int g = 0; g < NP; ++g) {
-      const int64_t xp = xa[g] < 0 ? 0 : xa[g];
-      const int64_t off = (xp * m + ro) * dpad + 8 * (lane >> 4);
+      const int64_t xrow = (xa[g] < 0 ? 0 : xa[g]) * m * (int64_t)dpad;
+      const _Float16 *ph = xh + xrow, *pl = xl + xrow;
 #pragma unroll
       for (int kb = 0; kb < KB; ++kb) {
-        Anh[g][kb] = *reinterpret_cast<const half8 *>(xh + off + kb * kKBlock);
-        Anl[g][kb] = *reinterpret_cast<const half8 *>(xl + off + kb * kKBlock);
+        Ah[g][kb] = *reinterpret_cast<const half8 *>(ph + lo + kb * kKBlock);
+        Al[g][kb] = *reinterpret_cast<const half8 *>(pl + lo + kb * kKBlock);
       }
     }
   }
-  __device__ __forceinline__ void promoteA() {
-#pragma unroll
-    for (int g = 0; g < NP; ++g)
-#pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
-        Ah[g][kb] = Anh[g][kb];
-        Al[g][kb] = Anl[g][kb];
-      }
-  }
-  __device__ __forceinline__ void fetchB(const PairGeom<G> &geo, int k) {
+  // B operands of chunk cB, then advance cB.
+  __device__ __forceinline__ void fetchB(const PairGeom<G> &geo) {
     const int lane = threadIdx.x & 63;
-    int sp, band, jv0;
-    geo.locate(k, sp, band, jv0);
-    const int jv = jv0 + (lane & 15);
-    int co = (jv - 1) >> geo.order;
+    if (cB.band == 0 && cB.jv0 == 0) {
+#pragma unroll
+      for (int q = 0; q < NB; ++q) yrow[q] = ypath(q, cB.sp) * n * dpad;
+    }
+    int co = (cB.jv0 + (lane & 15) - 1) >> geo.order;
     co = co < 0 ? 0 : (co > n - 1 ? n - 1 : co);
+    const uint32_t lo = (uint32_t)(co * dpad + 8 * (lane >> 4));
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
-      int64_t yp = ypath(q, sp);
-      yp = yp < ny ? yp : ny - 1;
-      const int64_t off = (yp * n + co) * dpad + 8 * (lane >> 4);
+      const _Float16 *ph = yh + yrow[q], *pl = yl + yrow[q];
 #pragma unroll
       for (int kb = 0; kb < KB; ++kb) {
-        Bnh[q][kb] = *reinterpret_cast<const half8 *>(yh + off + kb * kKBlock);
-        Bnl[q][kb] = *reinterpret_cast<const half8 *>(yl + off + kb * kKBlock);
+        Bh[q][kb] = *reinterpret_cast<const half8 *>(ph + lo + kb * kKBlock);
+        Bl[q][kb] = *reinterpret_cast<const half8 *>(pl + lo + kb * kKBlock);
       }
     }
+    cB.advance(geo);
   }
-  __device__ __forceinline__ void compute(const PairGeom<G> &geo, int k) {
-    const int lane = threadIdx.x & 63;
-    int sp, band, jv0;
-    geo.locate(k, sp, band, jv0);
-    if (jv0 == 0) promoteA();  // the chunk opens a band: its A operands were fetched ahead
-    // columns outside 1..N (the reset column and the padding) must read as c = 0
-    const int jv = jv0 + (lane & 15);
-    const bool ok = jv >= 1 && jv <= geo.N;
+  // Tiles of chunk cC, then advance cC; if the next chunk opens a band, load that band's A
+  // operands now, into the registers this chunk has just finished reading (they land while the
+  // wave computes).
+  __device__ __forceinline__ void compute(const PairGeom<G> &geo, bool more) {
+    if (cC.band == 0 && cC.jv0 == 0) {
 #pragma unroll
-    for (int q = 0; q < NB; ++q)
-#pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
-        const half8 z = {};
-        Bh[q][kb] = ok ? Bnh[q][kb] : z;
-        Bl[q][kb] = ok ? Bnl[q][kb] : z;
+      for (int g = 0; g < NP; ++g) {
+        const int64_t xp = xa[g] < 0 ? 0 : xa[g];
+        const int64_t yp = ypath(g, cC.sp);
+        float mm = kmult * invx[xp] * invy[yp];
+        if (sx) mm *= sx[xp];
+        if (sy) mm *= sy[yp];
+        pm[g] = mm;
       }
+    }
 #pragma unroll
     for (int g = 0; g < NP; ++g) {
       const int q = kSharedY ? 0 : g;
@@ -170,19 +170,14 @@ struct MfmaProducer {
       for (int kb = 0; kb < KB; ++kb)
         a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[g][kb], Bh[q][kb], a, 0, 0, 0);
       acc[g] = a;
-      const int64_t xp = xa[g] < 0 ? 0 : xa[g];
-      int64_t yp = ypath(g, sp);
-      yp = yp < ny ? yp : ny - 1;
-      float mm = kmult * invx[xp] * invy[yp];
-      if (sx) mm *= sx[xp];
-      if (sy) mm *= sy[yp];
-      mult[g] = mm;
+      mult[g] = pm[g];
     }
-    accJv0 = jv0;
+    cC.advance(geo);
+    if (more && cC.jv0 == 0) fetchA(geo, cC.band);
   }
-  __device__ __forceinline__ void store(float *ring, const PairGeom<G> &geo, int k) {
+  __device__ __forceinline__ void store(float *ring) {
     const int lane = threadIdx.x & 63;
-    const int slot0 = (k * kChunk) % Layout<G>::kRingW;
+    const int slot0 = slotS;
     const int c = lane & 15;
 #pragma unroll
     for (int g = 0; g < NP; ++g) {
@@ -197,30 +192,32 @@ struct MfmaProducer {
         *reinterpret_cast<float2 *>(mir + 2) = v1;
       }
     }
-  }
-  __device__ __forceinline__ void fetch_ahead(const PairGeom<G> &geo, int k) {
-    fetchB(geo, k);
-    int sp, band, jv0;
-    geo.locate(k, sp, band, jv0);
-    if (jv0 == 0) fetchA(geo, band);
+    slotS += kChunk;
+    if (slotS == Layout<G>::kRingW) slotS = 0;
   }
 
   __device__ __forceinline__ void prologue(float *ring, const PairGeom<G> &geo) {
     const int nchunks = geo.columns() / kChunk;
-    fetch_ahead(geo, 0);
-    compute(geo, 0);
-    store(ring, geo, 0);
+    cB.start(geo, 0);
+    cC.start(geo, 0);
+    slotS = 0;
+    fetchA(geo, 0);
+    fetchB(geo);
+    compute(geo, nchunks > 1);
+    store(ring);
     if (nchunks > 1) {
-      fetch_ahead(geo, 1);
-      compute(geo, 1);
+      fetchB(geo);
+      compute(geo, nchunks > 2);
     }
-    if (nchunks > 2) fetch_ahead(geo, 2);
+    if (nchunks > 2) fetchB(geo);
   }
-  __device__ __forceinline__ void begin_period(float *ring, const PairGeom<G> &geo, int per) {
+  // before the last consumer group of period p: write chunk p+1, multiply chunk p+2, fetch chunk
+  // p+3's B operands
+  __device__ __forceinline__ void period(float *ring, const PairGeom<G> &geo, int per) {
     const int nchunks = geo.columns() / kChunk;
-    if (per + 1 < nchunks) store(ring, geo, per + 1);
-    if (per + 2 < nchunks) compute(geo, per + 2);
-    if (per + 3 < nchunks) fetch_ahead(geo, per + 3);
+    if (per + 1 < nchunks) store(ring);
+    if (per + 2 < nchunks) compute(geo, per + 3 < nchunks);
+    if (per + 3 < nchunks) fetchB(geo);
   }
 };
 

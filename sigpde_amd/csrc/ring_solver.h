@@ -17,10 +17,13 @@
 //     (kChunk steps), so resets happen in a few known periods; the other periods skip the select.
 //   * Increments reach the lanes through an LDS ring of kRingW virtual columns x 64 rows (the P
 //     pairs' rows side by side), filled one chunk of kChunk columns ahead by a Producer (coalesced
-//     global loads here; the fused kernels produce with MFMA). Lanes read the ring on their skewed
+//     global loads here; the fused kernels produce with MFMA). The producer writes chunk p+1 just
+//     before the last consumer group of period p, when the oldest live column is t0+1, so the ring
+//     spans only the skew (G columns) plus one chunk. Lanes read the ring on their skewed
 //     diagonal; kRingMirror duplicated slots let a lane read kU consecutive columns without
-//     wrapping. The ring spans the skew (G columns) plus two chunks, so G = 16 needs half the LDS of
-//     G = 64 and lets twice as many waves share a CU.
+//     wrapping.
+//   * Columns outside 1..N of a band (the reset column and the padding) may hold any increment:
+//     the reset column's cell is replaced by 1 and padding cells only feed padding cells.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -31,7 +34,7 @@ namespace sigpde {
 constexpr int kWave = 64;
 constexpr int kChunk = 16;       // virtual columns per producer chunk (= steps per period)
 constexpr int kU = 4;            // consumer group: columns read per LDS batch
-constexpr int kRingMirror = 8;   // slots W..W+7 mirror slots 0..7
+constexpr int kRingMirror = 4;   // slots W..W+3 mirror slots 0..3
 constexpr int kRingStride = 66;  // elements per slot: 64 rows + 2 pad; conflict-free skewed reads
                                  // (bank = 2t - rho + 16g) and chunk writes (bank = 2col + row)
 
@@ -39,10 +42,11 @@ template <int G>
 struct Layout {
   static_assert(G == 16 || G == 64, "lanes per pair");
   static constexpr int kPairs = kWave / G;
-  static constexpr int kRingW = G == 64 ? 96 : 48;     // >= G + 2*kChunk, multiple of kChunk
+  static constexpr int kRingW = G == 64 ? 80 : 32;     // >= G + kChunk, multiple of kChunk
   static constexpr int kRingSlots = kRingW + kRingMirror;
   static constexpr int kMinBand = G == 64 ? 80 : 32;   // > G + 2*kU: bnd read after its write
-  static_assert(kRingW >= G + 2 * kChunk && kRingW % kChunk == 0, "ring geometry");
+  static_assert(kRingW >= G + kChunk && kRingW % kChunk == 0 && G % kChunk == 0,
+                "ring geometry");
   __host__ __device__ static int band_len(int N) {
     int np = N + 1 > kMinBand ? N + 1 : kMinBand;
     return (np + kChunk - 1) / kChunk * kChunk;
@@ -128,12 +132,15 @@ __device__ __forceinline__ double cell(double k01, double k10, double k00, doubl
   return r;
 }
 
-// LDS for one wave: the ring, then P band-boundary lines of Np doubles.
+// LDS for one wave: P band-boundary lines of Np doubles, then the ring (the lines come first so
+// that their per-group offsets fit the LDS instructions' immediate fields).
+template <int G>
+__host__ __device__ inline int bnd_stride(int N) { return Layout<G>::band_len(N); }
 template <Arith A, int G>
 __host__ __device__ inline size_t wave_lds_bytes(int N) {
   using L = Layout<G>;
   return (size_t)L::kRingSlots * kRingStride * sizeof(typename ArithTraits<A>::RT) +
-         (size_t)L::kPairs * L::band_len(N) * sizeof(double);
+         (size_t)L::kPairs * bnd_stride<G>(N) * sizeof(double);
 }
 
 // Geometry of a wave's virtual-column stream: `stream` consecutive pairs (all P lane groups of a
@@ -165,15 +172,42 @@ struct PairGeom {
   }
 };
 
-// Producer that reads increments (m x n, row stride ld) of type RT from global memory; lane group
+// Position of a chunk in the stream, advanced one chunk at a time (no divisions on the hot path).
+template <int G>
+struct Cursor {
+  int sp, band, jv0;  // stream pair, band, band column of the chunk's first virtual column
+  int slot0;          // ring slot of the chunk's first column
+  __device__ __forceinline__ void start(const PairGeom<G> &g, int k) {
+    g.locate(k, sp, band, jv0);
+    slot0 = (k * kChunk) % Layout<G>::kRingW;
+  }
+  __device__ __forceinline__ void advance(const PairGeom<G> &g) {
+    jv0 += kChunk;
+    if (jv0 == g.Np) {
+      jv0 = 0;
+      if (++band == g.nb) {
+        band = 0;
+        ++sp;
+      }
+    }
+    slot0 += kChunk;
+    if (slot0 == Layout<G>::kRingW) slot0 = 0;
+  }
+};
+
+// Producer interface (used by ring_solve):
+//   prologue(ring, geom): write chunk 0 and start chunk 1;
+//   period(ring, geom, p): called before the last consumer group of period p; writes chunk p+1.
+//
+// GlobalProducer reads increments (m x n, row stride ld) of type RT from global memory; lane group
 // g serves the pair whose increments start at inc[g] (null: an idle slot, served zeros). The
 // stream holds one pair per group.
 // Chunk k covers virtual columns [k*kChunk, (k+1)*kChunk) of one band (Np is a multiple of
 // kChunk): 64 ring rows (P pairs x G refined rows) x 16 columns. Load i (of 16) has lane l fetch
 // ring row 4i + (l >> 4) at column l & 15, i.e. four 64-byte row segments per load. load() only
-// issues the loads (clamped, always in bounds); masking and scaling happen in store(), one period
-// later, so the loads stay in flight while the wave computes. Rows past M read clamped data: they
-// only feed rows further down, never a row of the result.
+// issues the loads (indices clamped into the increment array); scaling happens in store(), one
+// period later, so the loads stay in flight while the wave computes. Rows past M and columns
+// outside 1..N read clamped data, which never reaches a result cell.
 template <typename RT, int G>
 struct GlobalProducer {
   using L = Layout<G>;
@@ -182,51 +216,50 @@ struct GlobalProducer {
   int64_t ld;
   bool scaled;
   RT buf[kChunk];
-  int pendJv0;
+
+  Cursor<G> cl;  // next chunk to load
+  int slotS;     // ring slot of the next chunk to store
 
   __device__ __forceinline__ void prologue(RT *ring, const PairGeom<G> &g) {
-    load(g, 0);
-    store(ring, g, 0);
-    if (g.columns() > kChunk) load(g, 1);
+    cl.start(g, 0);
+    slotS = 0;
+    load(g);
+    store(ring);
+    if (g.columns() > kChunk) load(g);
   }
-  __device__ __forceinline__ void begin_period(RT *ring, const PairGeom<G> &g, int per) {
+  __device__ __forceinline__ void period(RT *ring, const PairGeom<G> &g, int per) {
     const int nchunks = g.columns() / kChunk;
-    if (per + 1 < nchunks) store(ring, g, per + 1);
-    if (per + 2 < nchunks) load(g, per + 2);
+    if (per + 1 < nchunks) store(ring);
+    if (per + 2 < nchunks) load(g);
   }
 
-  __device__ __forceinline__ void load(const PairGeom<G> &g, int k) {
+  __device__ __forceinline__ void load(const PairGeom<G> &g) {
     const int lane = threadIdx.x & (kWave - 1);
-    int sp, b, jv0;
-    g.locate(k, sp, b, jv0);
+    const int b = cl.band, jv0 = cl.jv0;
     const int c = lane & 15;
     int jo = (jv0 + c - 1) >> g.order;
     jo = jo < 0 ? 0 : (jo > g.n - 1 ? g.n - 1 : jo);
-    const int bb = b < g.nb ? b : g.nb - 1;
 #pragma unroll
     for (int i = 0; i < kChunk; ++i) {
       const int rr = 4 * i + (lane >> 4);  // ring row
       const int pg = rr / G;               // pair slot
       const int rho = rr - pg * G;
-      int ro = (bb * G + rho) >> g.order;
+      int ro = (b * G + rho) >> g.order;
       ro = ro > g.m - 1 ? g.m - 1 : ro;
       const RT *base = inc[pg];
       buf[i] = base ? __builtin_nontemporal_load(base + (int64_t)ro * ld + jo) : (RT)0;
     }
-    pendJv0 = jv0;
+    cl.advance(g);
   }
-  __device__ __forceinline__ void store(RT *ring, const PairGeom<G> &g, int k) {
+  __device__ __forceinline__ void store(RT *ring) {
     const int lane = threadIdx.x & (kWave - 1);
     const int c = lane & 15;
-    const int slot0 = (k * kChunk) % L::kRingW;
-    const int jv = pendJv0 + c;
-    const bool ok = jv >= 1 && jv <= g.N;
+    const int slot0 = slotS;
     RT v[kChunk];
 #pragma unroll
     for (int i = 0; i < kChunk; ++i) {
       const int pg = (4 * i + (lane >> 4)) / G;
-      RT x = scaled ? buf[i] * scale[pg] : buf[i];
-      v[i] = ok ? x : (RT)0;
+      v[i] = scaled ? buf[i] * scale[pg] : buf[i];
     }
     RT *dst = ring + (slot0 + c) * kRingStride + (lane >> 4);
 #pragma unroll
@@ -236,70 +269,148 @@ struct GlobalProducer {
 #pragma unroll
       for (int i = 0; i < kChunk; ++i) mir[4 * i] = v[i];
     }
+    slotS += kChunk;
+    if (slotS == L::kRingW) slotS = 0;
   }
 };
 
-// One period (kChunk steps) of the consumer. kReset: lanes reach a reset column in this period;
-// kCapture: the result cell is computed in this period.
-template <Arith A, int G, bool kReset, bool kCapture>
-__device__ __forceinline__ void consume_period(const typename ArithTraits<A>::RT *ring,
-                                               double *bnd, const PairGeom<G> &g, double s,
-                                               int t0, int rst0, int jw0, bool top, int &slot,
-                                               double &k, double &kPrev, int tM, int rM,
-                                               double &result) {
+// Ring and boundary values of one consumer group (kU steps), loaded one group ahead.
+template <Arith A>
+struct GroupIn {
+  typename ArithTraits<A>::RT cv[kU];
+  double bv[kU];
+};
+
+// roff: byte offset (from the wave's LDS base) of this lane's ring element at the group's first
+// column; bp: the group's boundary values.
+template <Arith A>
+__device__ __forceinline__ void load_group(const char *lds, uint32_t roff, const double *bp,
+                                           GroupIn<A> &in) {
   using RT = typename ArithTraits<A>::RT;
-  using L = Layout<G>;
+  const RT *rp = reinterpret_cast<const RT *>(lds + roff);
+#pragma unroll
+  for (int q = 0; q < kU; ++q) in.cv[q] = rp[q * kRingStride];
+#pragma unroll
+  for (int q = 0; q < kU; ++q) in.bv[q] = bp[q];
+}
+
+// Consumer state carried from group to group.
+template <Arith A>
+struct ConsumerState {
+  GroupIn<A> nxt;   // inputs of the next group
+  uint32_t roff;    // ring byte offset of the group held in nxt
+  uint32_t rlim;    // roff wraps (by kRingW slots) when it reaches this
+  double k, kPrev;
+  double carry;     // the group's last lane: value of the previous group's last step
+};
+
+// Per-period boundary-line write plan: the group's last lane writes, one step late, the block of kU
+// columns starting at wp + kU*i for group i (at the group starting at t0 it holds columns
+// t0-(G-1)+q, q < kU, i.e. the block [t0-G, t0-G+kU) = {previous group's last value, this group's
+// first kU-1}). ones: the blocks belong to a pair's last band; they are written as ones, the next
+// pair's top boundary.
+struct WritePlan {
+  double *wp;
+  bool en;
+  bool ones;
+};
+
+// Consumer-period flavours: kReset (a lane may reach a reset column), kCapture (the result cell
+// may fall in the period), kOnes (the write plan may ask for ones). Plain periods have none.
+template <bool R, bool C, bool O>
+struct Flavour {
+  static constexpr bool kReset = R, kCapture = C, kOnes = O;
+};
+using Plain = Flavour<false, false, false>;
+using ResetOnly = Flavour<true, false, false>;
+using Special = Flavour<true, true, true>;
+
+// Group i (of 4) of a period: kU steps starting at t = t0 + 4i.
+//   bnext: boundary values the group prefetches for the next group; dl: rho - rst0 - 4i (the lane
+//   resets at step t + dl); tM, rM: result step and lane.
+template <Arith A, int G, class F>
+__device__ __forceinline__ void consume_group(const char *lds, const double *bnext, double s,
+                                              int t, int dl, double *wblk, const WritePlan &w,
+                                              GroupIn<A> &cur, ConsumerState<A> &st, int tM,
+                                              int rM, double &result) {
+  constexpr uint32_t kRingBytes = Layout<G>::kRingW * kRingStride * sizeof(typename ArithTraits<A>::RT);
+  constexpr uint32_t kGroupBytes = kU * kRingStride * sizeof(typename ArithTraits<A>::RT);
   const int lane = threadIdx.x & (kWave - 1);
   const int rho = lane & (G - 1);
-  const int grpBase = (lane / G) * g.Np;  // this lane's pair's bnd line
-  const int dl = rho - rst0;              // this lane resets at step t0 + dl
+  // prefetch the next group's inputs
+  st.roff += kGroupBytes;
+  st.roff = st.roff >= st.rlim ? st.roff - kRingBytes : st.roff;
+  load_group<A>(lds, st.roff, bnext, st.nxt);
+  double last[kU];
 #pragma unroll
-  for (int grp = 0; grp < kChunk; grp += kU) {
-    RT cv[kU];
-    const RT *rp = ring + slot * kRingStride + lane;
-#pragma unroll
-    for (int q = 0; q < kU; ++q) cv[q] = rp[q * kRingStride];
-    slot += kU;
-    if (slot >= L::kRingW) slot -= L::kRingW;
-    double bv[kU];
-#pragma unroll
-    for (int q = 0; q < kU; ++q) bv[q] = top ? 1.0 : bnd[grpBase + rst0 + grp + q];
-    double last[kU];
-#pragma unroll
-    for (int q = 0; q < kU; ++q) {
-      double cA, cB;
-      coefs<A>(cv[q], s, cA, cB);
-      const double up = shift_down<G>(k, bv[q]);
-      const double kn = cell<A>(up, k, kPrev, cA, cB);
-      kPrev = up;
-      k = kn;
-      if constexpr (kReset) {
-        const bool z = dl == grp + q;
-        k = z ? 1.0 : k;
-        kPrev = z ? 1.0 : kPrev;
-      }
-      if constexpr (kCapture) {
-        if (t0 + grp + q == tM) result = rho == rM ? k : result;
-      }
-      last[q] = k;
+  for (int q = 0; q < kU; ++q) {
+    double cA, cB;
+    coefs<A>(cur.cv[q], s, cA, cB);
+    const double up = shift_down<G>(st.k, cur.bv[q]);
+    const double kn = cell<A>(up, st.k, st.kPrev, cA, cB);
+    st.kPrev = up;
+    st.k = kn;
+    if constexpr (F::kReset) {
+      const bool z = dl == q;
+      st.k = z ? 1.0 : st.k;
+      st.kPrev = z ? 1.0 : st.kPrev;
     }
-    // the group's last lane parks its values (virtual columns t - (G-1)) for the next band
-    if (rho == G - 1) {
-      int j = jw0 + grp;
-#pragma unroll
-      for (int q = 0; q < kU; ++q) {
-        int jj = j + q;
-        jj = jj >= g.Np ? jj - g.Np : jj;
-        bnd[grpBase + jj] = last[q];
-      }
+    if constexpr (F::kCapture) {
+      if (t + q == tM) result = rho == rM ? st.k : result;
+    }
+    last[q] = st.k;
+  }
+  if (w.en && rho == G - 1) {
+    if (F::kOnes && w.ones) {
+      wblk[0] = 1.0;
+      wblk[1] = 1.0;
+      wblk[2] = 1.0;
+      wblk[3] = 1.0;
+    } else {
+      wblk[0] = st.carry;
+      wblk[1] = last[0];
+      wblk[2] = last[1];
+      wblk[3] = last[2];
     }
   }
+  st.carry = last[kU - 1];
+}
+
+// Groups 0..2 of a period (rst0 = t0 mod Np: the group's first lane's band column, and the lane
+// that resets at t0). Group i prefetches boundary values at bndLine + rst0 + kU*(i+1).
+template <Arith A, int G, class F>
+__device__ __forceinline__ void consume_head(const char *lds, const double *bndLine, double s,
+                                             int t0, int rst0, const WritePlan &w,
+                                             ConsumerState<A> &st, int tM, int rM,
+                                             double &result) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int dl = (lane & (G - 1)) - rst0;
+  const double *bp = bndLine + rst0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    GroupIn<A> cur = st.nxt;
+    consume_group<A, G, F>(lds, bp + kU * (i + 1), s, t0 + kU * i, dl - kU * i, w.wp + kU * i, w,
+                           cur, st, tM, rM, result);
+  }
+}
+
+// Group 3 of a period; it prefetches the next period's first boundary block at bndLine + jnext.
+template <Arith A, int G, class F>
+__device__ __forceinline__ void consume_tail(const char *lds, const double *bndLine, double s,
+                                             int t0, int rst0, int jnext, const WritePlan &w,
+                                             ConsumerState<A> &st, int tM, int rM,
+                                             double &result) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int dl = (lane & (G - 1)) - rst0;
+  GroupIn<A> cur = st.nxt;
+  consume_group<A, G, F>(lds, bndLine + jnext, s, t0 + 3 * kU, dl - 3 * kU, w.wp + 3 * kU, w, cur,
+                         st, tM, rM, result);
 }
 
 // Solves the wave's stream: g.stream pairs per lane group, one after another. For every pair of
 // the stream, out(sp, value) is called by the lanes rho == (M-1) mod G of each group with that
 // group's k[M][N] (sp = position in the stream).
-//   P: producer with prologue(ring, geom) / begin_period(ring, geom, period)
+//   P: producer (interface above)
 //   s: the per-lane double scale of the Scaled* modes (ignored otherwise)
 //   lds: this wave's LDS (wave_lds_bytes<A, G>(N) bytes)
 template <Arith A, int G, class Producer, class Out>
@@ -308,12 +419,16 @@ __device__ void ring_solve(Producer &P, const PairGeom<G> &g, double s, void *ld
   using L = Layout<G>;
   const int lane = threadIdx.x & (kWave - 1);
   const int rho = lane & (G - 1);
-  RT *ring = reinterpret_cast<RT *>(lds);
-  double *bnd = reinterpret_cast<double *>(ring + L::kRingSlots * kRingStride);
+  const int bs = bnd_stride<G>(g.N);
+  double *bnd = reinterpret_cast<double *>(lds);
+  RT *ring = reinterpret_cast<RT *>(bnd + L::kPairs * bs);
+  const char *ldsb = reinterpret_cast<const char *>(lds);
+  const uint32_t ringOff = (uint32_t)(L::kPairs * bs * sizeof(double));
+  double *bndLine = bnd + (lane / G) * bs;  // this lane's pair's boundary line
 
   // zero the ring (columns not yet produced read as c = 0) and set the boundary lines to 1
   for (int i = lane; i < L::kRingSlots * kRingStride; i += kWave) ring[i] = (RT)0;
-  for (int i = lane; i < L::kPairs * g.Np; i += kWave) bnd[i] = 1.0;
+  for (int i = lane; i < L::kPairs * bs; i += kWave) bnd[i] = 1.0;
 
   const int total = g.columns();            // virtual columns
   const int nsteps = total + G - 1;         // last lane finishes here
@@ -328,37 +443,61 @@ __device__ void ring_solve(Producer &P, const PairGeom<G> &g, double s, void *ld
 
   P.prologue(ring, g);
 
-  double k = 1.0, kPrev = 1.0;
-  int slot = (L::kRingW - rho) % L::kRingW;  // ring slot at step 0 (virtual column -rho)
-  int rst0 = 0;                              // t0 mod Np: the group's first lane's band column
-  int jw0 = g.Np - (G - 1);                  // (t0 - (G-1)) mod Np: the group's last lane's
-  int band0 = 0;                             // band of the group's first lane
+  ConsumerState<A> st;
+  st.k = 1.0;
+  st.kPrev = 1.0;
+  st.carry = 1.0;
+  st.rlim = ringOff + (uint32_t)((L::kRingW * kRingStride + lane) * sizeof(RT));
+  // ring slot at step 0 is virtual column -rho
+  st.roff = ringOff + (uint32_t)((((L::kRingW - rho) % L::kRingW) * kRingStride + lane) * sizeof(RT));
+  load_group<A>(ldsb, st.roff, bndLine, st.nxt);
+  int rst0 = 0;  // t0 mod Np: the group's first lane's band column
+  // The group starting at step t writes the boundary block of virtual column t - G (nothing while
+  // that is negative). G is a multiple of kChunk, so a period writes either nothing or four blocks
+  // of one band, at band column wjv of band wband (tracked without divisions).
+  int vw = -G;
+  int wjv = 0, wband = 0;
 
   for (int per = 0; per < nperiods; ++per) {
-    P.begin_period(ring, g, per);
     const int t0 = per * kChunk;
-    const bool top = band0 == 0;
+    WritePlan w;
+    w.en = vw >= 0;
+    w.wp = bndLine + wjv;
+    w.ones = w.en && wband == g.nb - 1;
+    if (w.en) {
+      wjv += kChunk;
+      if (wjv == g.Np) {
+        wjv = 0;
+        if (++wband == g.nb) wband = 0;
+      }
+    }
+    int jnext = rst0 + kChunk;
+    jnext = jnext == g.Np ? 0 : jnext;
     double result = 0.0;
-    if (sp < g.stream && tM >= t0 && tM < t0 + kChunk) {
-      consume_period<A, G, true, true>(ring, bnd, g, s, t0, rst0, jw0, top, slot, k, kPrev, tM,
-                                       rM, result);
-      if (rho == rM) out(sp, result);
-      ++sp;
-      tM += pc;
-    } else if (rst0 < G) {
-      consume_period<A, G, true, false>(ring, bnd, g, s, t0, rst0, jw0, top, slot, k, kPrev, tM,
-                                        rM, result);
+    const bool cap = sp < g.stream && tM >= t0 && tM < t0 + kChunk;
+    const int fl = (cap || w.ones) ? 2 : (rst0 < G ? 1 : 0);
+    if (fl == 2)
+      consume_head<A, G, Special>(ldsb, bndLine, s, t0, rst0, w, st, tM, rM, result);
+    else if (fl == 1)
+      consume_head<A, G, ResetOnly>(ldsb, bndLine, s, t0, rst0, w, st, tM, rM, result);
+    else
+      consume_head<A, G, Plain>(ldsb, bndLine, s, t0, rst0, w, st, tM, rM, result);
+    P.period(ring, g, per);
+    if (fl == 2) {
+      consume_tail<A, G, Special>(ldsb, bndLine, s, t0, rst0, jnext, w, st, tM, rM, result);
+      if (cap) {
+        if (rho == rM) out(sp, result);
+        ++sp;
+        tM += pc;
+      }
+    } else if (fl == 1) {
+      consume_tail<A, G, ResetOnly>(ldsb, bndLine, s, t0, rst0, jnext, w, st, tM, rM, result);
     } else {
-      consume_period<A, G, false, false>(ring, bnd, g, s, t0, rst0, jw0, top, slot, k, kPrev, tM,
-                                         rM, result);
+      consume_tail<A, G, Plain>(ldsb, bndLine, s, t0, rst0, jnext, w, st, tM, rM, result);
     }
     rst0 += kChunk;
-    if (rst0 == g.Np) {
-      rst0 = 0;
-      if (++band0 == g.nb) band0 = 0;
-    }
-    jw0 += kChunk;
-    if (jw0 >= g.Np) jw0 -= g.Np;
+    if (rst0 == g.Np) rst0 = 0;
+    vw += kChunk;
   }
 }
 
