@@ -7,22 +7,19 @@
 // Precision "fp16x2": each path's increments are formed exactly in double, scaled by a per-path
 // power of two 2^e (so max |dx| * 2^e lies in [2^13, 2^14)) and split into fp16 hi + lo. Products
 // lo*hi + hi*lo + hi*hi (the lo*lo term is below the fp16x2 resolution) accumulate in fp32 on
-// v_mfma_f32_16x16x32_f16, are scaled back in fp32, and the wavefront runs in double
-// (Arith::Mixed). Relative error against the float64 reference: ~1e-7 on Brownian paths.
+// v_mfma_f32_16x16x32_f16 and go to the ring unscaled; the wavefront runs in double and applies
+// the pair's scale m = kscale 4^-order 2^-(ex+ey) (x_scale y_scale) through its coefficients
+// (Arith::Mixed). Relative error against the float64 reference: ~1e-6 on Brownian paths.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
 
 #include "capi_util.h"
+#include "mfma_producer.h"
 #include "ring_solver.h"
 
 namespace sigpde {
-
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef float float4v __attribute__((ext_vector_type(4)));
-
-constexpr int kKBlock = 32;  // K of one v_mfma_f32_16x16x32_f16
 
 // ------------------------------------------------------------------ split prepass
 // One block per path: dx = x[s+1] - x[s] in double (exact for float inputs), per-path scale,
@@ -64,181 +61,13 @@ __global__ __launch_bounds__(256) void k_split_fp16(const T *__restrict__ X, int
   }
 }
 
-// ------------------------------------------------------------------ MFMA producer (G = 16)
-// Lane group g (16 lanes) solves pairs (x path xa[g], y path ybase[g] + sp * ystep) of the stream.
-// Chunk k of the stream covers 16 refined columns of one band (16 refined rows) of every group's
-// pair: one 16x16 MFMA tile per group, D[row][col] with A = the band's x-increment rows and
-// B = the chunk's y-increment columns (refined indices repeat original rows / columns 2^order
-// times; columns outside 1..N read a clamped column, see ring_solver.h). Lane l holds
-// D[4(l/16) + e][l % 16] and writes those four ring rows of column l % 16.
-// Pipeline, run before the last consumer group of period p: write chunk p+1's tiles, issue chunk
-// p+2's MFMAs, fetch chunk p+3's B operands; after the last chunk of a band is multiplied, the
-// next band's A operands are loaded. Operand rows are addressed as a wave-uniform 64-bit row base
-// plus a 32-bit per-lane offset.
-template <int KB, bool kSharedY>
-struct MfmaProducer {
-  static constexpr int G = 16;
-  static constexpr int NP = 4;           // pairs (lane groups) per wave
-  static constexpr int NB = kSharedY ? 1 : NP;
-  const _Float16 *xh, *xl, *yh, *yl;
-  int dpad;
-  int m, n;                              // original increments per x / y path
-  int64_t xa[NP];                        // x path of each group (-1: idle)
-  int64_t ybase[NP];
-  int ystep;
-  const float *invx, *invy;
-  float kmult;                           // kscale / 4^order
-  const float *sx, *sy;                  // optional extra scales (robust Gram), may be null
-  int64_t ny;                            // valid y paths (ybase + sp*ystep < ny)
-
-  half8 Ah[NP][KB], Al[NP][KB];          // A operands of the band being multiplied
-  half8 Bh[NB][KB], Bl[NB][KB];          // next chunk's B operands (fetched one period ahead)
-  float4v acc[NP];
-  float mult[NP];                        // scale of the tiles in acc
-  float pm[NP];                          // scale of the pair being multiplied (per group)
-  int64_t yrow[NB];                      // first B row (in halves) of the pair being fetched
-  Cursor<G> cB, cC;                      // next chunk to fetch B for / to multiply
-  int slotS;                             // ring slot of the next chunk to store
-
-  __device__ __forceinline__ int64_t ypath(int g, int sp) const {
-    const int64_t yp = ybase[g] + (int64_t)sp * ystep;
-    return yp < ny ? yp : ny - 1;
-  }
-
-  __device__ __forceinline__ void fetchA(const PairGeom<G> &geo, int band) {
-    const int lane = threadIdx.x & 63;
-    int ro = (band * G + (lane & 15)) >> geo.order;
-    ro = ro > m - 1 ? m - 1 : ro;
-    const uint32_t lo = (uint32_t)(ro * dpad + 8 * (lane >> 4));
-#pragma unroll
-    for (int g = 0; g < NP; ++g) {
-      const int64_t xrow = (xa[g] < 0 ? 0 : xa[g]) * m * (int64_t)dpad;
-      const _Float16 *ph = xh + xrow, *pl = xl + xrow;
-#pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
-        Ah[g][kb] = *reinterpret_cast<const half8 *>(ph + lo + kb * kKBlock);
-        Al[g][kb] = *reinterpret_cast<const half8 *>(pl + lo + kb * kKBlock);
-      }
-    }
-  }
-  // B operands of chunk cB, then advance cB.
-  __device__ __forceinline__ void fetchB(const PairGeom<G> &geo) {
-    const int lane = threadIdx.x & 63;
-    if (cB.band == 0 && cB.jv0 == 0) {
-#pragma unroll
-      for (int q = 0; q < NB; ++q) yrow[q] = ypath(q, cB.sp) * n * dpad;
-    }
-    int co = (cB.jv0 + (lane & 15) - 1) >> geo.order;
-    co = co < 0 ? 0 : (co > n - 1 ? n - 1 : co);
-    const uint32_t lo = (uint32_t)(co * dpad + 8 * (lane >> 4));
-#pragma unroll
-    for (int q = 0; q < NB; ++q) {
-      const _Float16 *ph = yh + yrow[q], *pl = yl + yrow[q];
-#pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
-        Bh[q][kb] = *reinterpret_cast<const half8 *>(ph + lo + kb * kKBlock);
-        Bl[q][kb] = *reinterpret_cast<const half8 *>(pl + lo + kb * kKBlock);
-      }
-    }
-    cB.advance(geo);
-  }
-  // Tiles of chunk cC, then advance cC; if the next chunk opens a band, load that band's A
-  // operands now, into the registers this chunk has just finished reading (they land while the
-  // wave computes).
-  __device__ __forceinline__ void compute(const PairGeom<G> &geo, bool more) {
-    if (cC.band == 0 && cC.jv0 == 0) {
-#pragma unroll
-      for (int g = 0; g < NP; ++g) {
-        const int64_t xp = xa[g] < 0 ? 0 : xa[g];
-        const int64_t yp = ypath(g, cC.sp);
-        float mm = kmult * invx[xp] * invy[yp];
-        if (sx) mm *= sx[xp];
-        if (sy) mm *= sy[yp];
-        pm[g] = mm;
-      }
-    }
-#pragma unroll
-    for (int g = 0; g < NP; ++g) {
-      const int q = kSharedY ? 0 : g;
-      float4v a = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
-        a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al[g][kb], Bh[q][kb], a, 0, 0, 0);
-        a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[g][kb], Bl[q][kb], a, 0, 0, 0);
-      }
-#pragma unroll
-      for (int kb = 0; kb < KB; ++kb)
-        a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[g][kb], Bh[q][kb], a, 0, 0, 0);
-      acc[g] = a;
-      mult[g] = pm[g];
-    }
-    cC.advance(geo);
-    if (more && cC.jv0 == 0) fetchA(geo, cC.band);
-  }
-  __device__ __forceinline__ void store(float *ring) {
-    const int lane = threadIdx.x & 63;
-    const int slot0 = slotS;
-    const int c = lane & 15;
-#pragma unroll
-    for (int g = 0; g < NP; ++g) {
-      float *dst = ring + (slot0 + c) * kRingStride + g * G + 4 * (lane >> 4);
-      float2 v0 = make_float2(acc[g][0] * mult[g], acc[g][1] * mult[g]);
-      float2 v1 = make_float2(acc[g][2] * mult[g], acc[g][3] * mult[g]);
-      *reinterpret_cast<float2 *>(dst) = v0;
-      *reinterpret_cast<float2 *>(dst + 2) = v1;
-      if (slot0 == 0 && c < kRingMirror) {
-        float *mir = ring + (Layout<G>::kRingW + c) * kRingStride + g * G + 4 * (lane >> 4);
-        *reinterpret_cast<float2 *>(mir) = v0;
-        *reinterpret_cast<float2 *>(mir + 2) = v1;
-      }
-    }
-    slotS += kChunk;
-    if (slotS == Layout<G>::kRingW) slotS = 0;
-  }
-
-  __device__ __forceinline__ void prologue(float *ring, const PairGeom<G> &geo) {
-    const int nchunks = geo.columns() / kChunk;
-    cB.start(geo, 0);
-    cC.start(geo, 0);
-    slotS = 0;
-    fetchA(geo, 0);
-    fetchB(geo);
-    compute(geo, nchunks > 1);
-    store(ring);
-    if (nchunks > 1) {
-      fetchB(geo);
-      compute(geo, nchunks > 2);
-    }
-    if (nchunks > 2) fetchB(geo);
-  }
-  // before the last consumer group of period p: write chunk p+1, multiply chunk p+2, fetch chunk
-  // p+3's B operands
-  __device__ __forceinline__ void period(float *ring, const PairGeom<G> &geo, int per) {
-    const int nchunks = geo.columns() / kChunk;
-    if (per + 1 < nchunks) store(ring);
-    if (per + 2 < nchunks) compute(geo, per + 3 < nchunks);
-    if (per + 3 < nchunks) fetchB(geo);
-  }
-};
-
 // ------------------------------------------------------------------ kernels
 // Gram: a wave takes four x paths (a quad) and streams `ys` y paths through them.
 // Symmetric: only y <= x pairs are written, with their mirror.
-template <typename TO>
-struct GramFusedArgs {
-  const _Float16 *xh, *xl, *yh, *yl;
-  const float *invx, *invy;
-  int64_t Bx, By;
-  int m, n, order, dpad, ys;
-  float kmult;
-  int symmetric;
-  TO *out;
-  int64_t ldo;
-};
 
 template <typename TO, int KB>
 __global__ __launch_bounds__(64) void k_linear_gram_mixed(GramFusedArgs<TO> a,
-                                                          const float *sxf, const float *syf) {
+                                                          const double *sxd, const double *syd) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int G = 16;
   const int lane = threadIdx.x & 63;
@@ -270,11 +99,8 @@ __global__ __launch_bounds__(64) void k_linear_gram_mixed(GramFusedArgs<TO> a,
   P.m = a.m;
   P.n = a.n;
   P.ystep = 1;
-  P.invx = a.invx;
   P.invy = a.invy;
-  P.kmult = a.kmult;
-  P.sx = sxf;
-  P.sy = syf;
+  P.sy = syd;
   P.ny = a.By;
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
@@ -283,6 +109,12 @@ __global__ __launch_bounds__(64) void k_linear_gram_mixed(GramFusedArgs<TO> a,
     P.ybase[g] = y0;
   }
   const int64_t xme = xq * 4 + lane / G;
+  {
+    const int64_t xc = xme < a.Bx ? xme : a.Bx - 1;
+    P.xsl = a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0);
+    P.ysl = 1.0;
+    P.pmY = 1.0;
+  }
   ring_solve<Arith::Mixed, G>(P, geo, 1.0, smem_raw, [&](int sp, double r) {
     const int64_t y = y0 + sp;
     if (xme >= a.Bx || y >= a.By) return;
@@ -296,53 +128,18 @@ __global__ __launch_bounds__(64) void k_linear_gram_mixed(GramFusedArgs<TO> a,
   });
 }
 
-// Pairwise: a wave solves four pairs (x_b, y_b).
-template <typename TO, int KB>
-__global__ __launch_bounds__(64) void k_linear_pairwise_mixed(GramFusedArgs<TO> a,
-                                                              const float *sxf,
-                                                              const float *syf) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  constexpr int G = 16;
-  const int lane = threadIdx.x & 63;
-  PairGeom<G> geo(a.m, a.n, a.order, 1);
-  MfmaProducer<KB, false> P;
-  P.xh = a.xh;
-  P.xl = a.xl;
-  P.yh = a.yh;
-  P.yl = a.yl;
-  P.dpad = a.dpad;
-  P.m = a.m;
-  P.n = a.n;
-  P.ystep = 0;
-  P.invx = a.invx;
-  P.invy = a.invy;
-  P.kmult = a.kmult;
-  P.sx = sxf;
-  P.sy = syf;
-  P.ny = a.By;
-#pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    const int64_t b = (int64_t)blockIdx.x * 4 + g;
-    P.xa[g] = b < a.Bx ? b : -1;
-    P.ybase[g] = b < a.By ? b : a.By - 1;
-  }
-  const int64_t bme = (int64_t)blockIdx.x * 4 + lane / G;
-  ring_solve<Arith::Mixed, G>(P, geo, 1.0, smem_raw, [&](int, double r) {
-    if (bme < a.Bx) a.out[bme] = (TO)r;
-  });
-}
-
-// float copies of optional scales (the fused producer works in float)
+// double copies of optional scales
 template <typename T>
-__global__ void k_to_float(const T *src, int64_t n, float *dst) {
+__global__ void k_to_double(const T *src, int64_t n, double *dst) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = (float)src[i];
+  if (i < n) dst[i] = (double)src[i];
 }
 
 // ------------------------------------------------------------------ host side
 struct Workspace {
   _Float16 *xh, *xl, *yh, *yl;
-  float *invx, *invy, *sxf, *syf;
+  float *invx, *invy;
+  double *sxd, *syd;
 };
 
 static inline size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -366,8 +163,8 @@ static size_t workspace_layout(int64_t Bx, int64_t Lx, int64_t By, int64_t Ly, i
   t.yl = same ? t.xl : (_Float16 *)take(ny * 2);
   t.invx = (float *)take(Bx * 4);
   t.invy = same ? t.invx : (float *)take(By * 4);
-  t.sxf = (float *)take(Bx * 4);
-  t.syf = (float *)take(By * 4);
+  t.sxd = (double *)take(Bx * 8);
+  t.syd = (double *)take(By * 8);
   if (w) *w = t;
   return off;
 }
@@ -381,9 +178,9 @@ static hipError_t run_split(const T *X, int64_t B, int64_t L, int64_t d, int dp,
 }
 
 template <typename T>
-static hipError_t run_to_float(const void *src, int64_t n, float *dst, hipStream_t st) {
+static hipError_t run_to_double(const void *src, int64_t n, double *dst, hipStream_t st) {
   if (!src || n == 0) return hipSuccess;
-  k_to_float<T><<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>((const T *)src, n, dst);
+  k_to_double<T><<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>((const T *)src, n, dst);
   return hipGetLastError();
 }
 
@@ -399,8 +196,8 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
     e = run_split<T>((const T *)Y, By, Ly, d, dp, w.yh, w.yl, w.invy, st);
     if (e != hipSuccess) return e;
   }
-  if ((e = run_to_float<T>(sx, Bx, w.sxf, st)) != hipSuccess) return e;
-  if ((e = run_to_float<T>(sy, By, w.syf, st)) != hipSuccess) return e;
+  if ((e = run_to_double<T>(sx, Bx, w.sxd, st)) != hipSuccess) return e;
+  if ((e = run_to_double<T>(sy, By, w.syd, st)) != hipSuccess) return e;
   GramFusedArgs<T> a{};
   a.xh = w.xh;
   a.xl = w.xl;
@@ -414,16 +211,15 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
   a.n = (int)(Ly - 1);
   a.order = order;
   a.dpad = dp;
-  a.kmult = (float)(kscale / (double)(1ll << (2 * order)));
+  a.kmult = kscale / (double)(1ll << (2 * order));
   a.symmetric = symmetric;
   a.out = (T *)out;
   a.ldo = ldo;
-  const float *sxf = sx ? w.sxf : nullptr;
-  const float *syf = sy ? w.syf : nullptr;
+  const double *sxd = sx ? w.sxd : nullptr;
+  const double *syd = sy ? w.syd : nullptr;
   const size_t shmem = wave_lds_bytes<Arith::Mixed, 16>((int)((Ly - 1) << order));
   if (pairwise) {
-    const unsigned blocks = (unsigned)((Bx + 3) / 4);
-    k_linear_pairwise_mixed<T, KB><<<dim3(blocks), dim3(64), shmem, st>>>(a, sxf, syf);
+    return launch_linear_pairwise_mixed<T, KB>(a, sxd, syd, shmem, st);
   } else {
     // y paths streamed per wave: enough waves to fill the GPU, long enough streams to amortise
     // the 15-step pipeline fill of each pair
@@ -434,7 +230,7 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
     const int64_t nblk = nq * ((By + ys - 1) / ys);
     // a multiple of 8 blocks so that the XCD-major block order is a bijection onto the work units
     const int64_t grid = (nblk + 7) / 8 * 8;
-    k_linear_gram_mixed<T, KB><<<dim3((unsigned)grid), dim3(64), shmem, st>>>(a, sxf, syf);
+    k_linear_gram_mixed<T, KB><<<dim3((unsigned)grid), dim3(64), shmem, st>>>(a, sxd, syd);
   }
   return hipGetLastError();
 }

@@ -85,7 +85,9 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 // Ref64: ring holds double c; all double.
 // Scaled32 / Scaled64: ring holds the raw increment; c = double(inc) * s with a per-lane double s
 //        (the root-find's inc * xt**2), cells rounded to float for Scaled32.
-// Mixed: ring holds float c from the split-precision increment Gram; double arithmetic and state.
+// Mixed: ring holds the raw float increment product p of the split-precision increment Gram; the
+//        pair's scale m (c = m * p) is folded into per-lane double coefficients h = m/2 and
+//        q = m^2/12; double arithmetic and state.
 enum class Arith { Ref32, Ref64, Scaled32, Scaled64, Mixed };
 
 template <Arith A>
@@ -104,10 +106,17 @@ struct ArithTraits<Arith::Mixed> { using RT = float; static constexpr bool kRoun
 // A = (1 + c/2) + c^2/12 and B = 1 - c^2/12 as pde_step writes them (the compiler may contract the
 // products into FMAs, as NVVM does for the reference).
 template <Arith A>
-__device__ __forceinline__ void coefs(typename ArithTraits<A>::RT c, double s, double &cA,
-                                      double &cB) {
+__device__ __forceinline__ void coefs(typename ArithTraits<A>::RT c, double s, double h, double q,
+                                      double &cA, double &cB) {
   double cd, c2;
-  if constexpr (A == Arith::Ref32) {
+  if constexpr (A == Arith::Mixed) {
+    // A = 1 + (m/2) p + (m^2/12) p^2, B = 1 - (m^2/12) p^2 with c = m p
+    cd = (double)c;
+    c2 = cd * cd;
+    cA = fma(q, c2, fma(cd, h, 1.0));
+    cB = fma(-q, c2, 1.0);
+    return;
+  } else if constexpr (A == Arith::Ref32) {
     float c2f = c * c;
     cd = (double)c;
     c2 = (double)c2f;
@@ -176,11 +185,7 @@ struct PairGeom {
 template <int G>
 struct Cursor {
   int sp, band, jv0;  // stream pair, band, band column of the chunk's first virtual column
-  int slot0;          // ring slot of the chunk's first column
-  __device__ __forceinline__ void start(const PairGeom<G> &g, int k) {
-    g.locate(k, sp, band, jv0);
-    slot0 = (k * kChunk) % Layout<G>::kRingW;
-  }
+  __device__ __forceinline__ void start(const PairGeom<G> &g, int k) { g.locate(k, sp, band, jv0); }
   __device__ __forceinline__ void advance(const PairGeom<G> &g) {
     jv0 += kChunk;
     if (jv0 == g.Np) {
@@ -190,13 +195,12 @@ struct Cursor {
         ++sp;
       }
     }
-    slot0 += kChunk;
-    if (slot0 == Layout<G>::kRingW) slot0 = 0;
   }
 };
 
 // Producer interface (used by ring_solve):
 //   prologue(ring, geom): write chunk 0 and start chunk 1;
+//   early(ring, geom, p): called at the start of period p (before the consumer's first group);
 //   period(ring, geom, p): called before the last consumer group of period p; writes chunk p+1.
 //
 // GlobalProducer reads increments (m x n, row stride ld) of type RT from global memory; lane group
@@ -227,6 +231,7 @@ struct GlobalProducer {
     store(ring);
     if (g.columns() > kChunk) load(g);
   }
+  __device__ __forceinline__ void early(RT *, const PairGeom<G> &, int) {}
   __device__ __forceinline__ void period(RT *ring, const PairGeom<G> &g, int per) {
     const int nchunks = g.columns() / kChunk;
     if (per + 1 < nchunks) store(ring);
@@ -302,6 +307,8 @@ struct ConsumerState {
   uint32_t rlim;    // roff wraps (by kRingW slots) when it reaches this
   double k, kPrev;
   double carry;     // the group's last lane: value of the previous group's last step
+  double h, q;      // Mixed: coefficients of the lane's current pair
+  double hN, qN;    // Mixed: coefficients taken at the lane's next reset
 };
 
 // Per-period boundary-line write plan: the group's last lane writes, one step late, the block of kU
@@ -345,7 +352,7 @@ __device__ __forceinline__ void consume_group(const char *lds, const double *bne
 #pragma unroll
   for (int q = 0; q < kU; ++q) {
     double cA, cB;
-    coefs<A>(cur.cv[q], s, cA, cB);
+    coefs<A>(cur.cv[q], s, st.h, st.q, cA, cB);
     const double up = shift_down<G>(st.k, cur.bv[q]);
     const double kn = cell<A>(up, st.k, st.kPrev, cA, cB);
     st.kPrev = up;
@@ -354,6 +361,10 @@ __device__ __forceinline__ void consume_group(const char *lds, const double *bne
       const bool z = dl == q;
       st.k = z ? 1.0 : st.k;
       st.kPrev = z ? 1.0 : st.kPrev;
+      if constexpr (A == Arith::Mixed) {
+        st.h = z ? st.hN : st.h;
+        st.q = z ? st.qN : st.q;
+      }
     }
     if constexpr (F::kCapture) {
       if (t + q == tM) result = rho == rM ? st.k : result;
@@ -410,7 +421,9 @@ __device__ __forceinline__ void consume_tail(const char *lds, const double *bndL
 // Solves the wave's stream: g.stream pairs per lane group, one after another. For every pair of
 // the stream, out(sp, value) is called by the lanes rho == (M-1) mod G of each group with that
 // group's k[M][N] (sp = position in the stream).
-//   P: producer (interface above)
+//   P: producer (interface above); for Arith::Mixed it also provides lane_scale(), the scale m of
+//      the pair whose chunk it multiplied last (read at the start of the period that holds a
+//      pair's first reset, before early(): the last chunk multiplied then is that pair's first)
 //   s: the per-lane double scale of the Scaled* modes (ignored otherwise)
 //   lds: this wave's LDS (wave_lds_bytes<A, G>(N) bytes)
 template <Arith A, int G, class Producer, class Out>
@@ -447,6 +460,9 @@ __device__ void ring_solve(Producer &P, const PairGeom<G> &g, double s, void *ld
   st.k = 1.0;
   st.kPrev = 1.0;
   st.carry = 1.0;
+  st.h = st.q = st.hN = st.qN = 0.0;
+  static_assert(A != Arith::Mixed || G == kChunk, "Mixed: one reset period per band");
+  int rband = 0;  // band whose reset column the next reset period holds
   st.rlim = ringOff + (uint32_t)((L::kRingW * kRingStride + lane) * sizeof(RT));
   // ring slot at step 0 is virtual column -rho
   st.roff = ringOff + (uint32_t)((((L::kRingW - rho) % L::kRingW) * kRingStride + lane) * sizeof(RT));
@@ -473,6 +489,17 @@ __device__ void ring_solve(Producer &P, const PairGeom<G> &g, double s, void *ld
     }
     int jnext = rst0 + kChunk;
     jnext = jnext == g.Np ? 0 : jnext;
+    if constexpr (A == Arith::Mixed) {
+      if (rst0 == 0) {
+        if (rband == 0) {
+          const double m = P.lane_scale();
+          st.hN = 0.5 * m;
+          st.qN = m * m * (1.0 / 12);
+        }
+        if (++rband == g.nb) rband = 0;
+      }
+    }
+    P.early(ring, g, per);
     double result = 0.0;
     const bool cap = sp < g.stream && tM >= t0 && tM < t0 + kChunk;
     const int fl = (cap || w.ones) ? 2 : (rst0 < G ? 1 : 0);
