@@ -3,12 +3,17 @@
 Workload (default): Gram of X, Y each 1024 Brownian paths, length 256, dim 64, dyadic order 0,
 linear static kernel (increments N(0, 1/256), seeds 5/6, float32 inputs resident in HBM).
 One step = one full 1024 x 1024 Gram. value = pairs/sec = 1024^2 / step time of the
-mixed-precision path (precision="fp16x2": fp16 hi/lo increment Gram on MFMA, double wavefront);
-the plain FP32 drop-in (precision="ref", the reference's arithmetic) is timed beside it.
+mixed-precision path (precision="fp16x2": fp16 hi/lo increment Gram on MFMA, double wavefront).
+Timed beside it on the same inputs:
+  plain_fp32             precision="fp32": the same fused design in plain FP32 (fp32 increments on
+                         fp32 MFMA, the reference's float32 wavefront typing)
+  reference_scheme_fp32  precision="ref": the reference's algorithm in float32 (torch materialises
+                         each increment tile from the point Gram, then the generic wavefront kernel)
 
 Besides the JSON contract fields the line carries:
   rel_err_fp64   ||K - K64||_F / ||K64||_F on a sampled block, K64 = the float64 oracle on the same
-                 float32 inputs upcast (computed in the CPU-baseline leg, which runs the oracle)
+                 float32 inputs upcast (computed in the CPU-baseline leg, which runs the oracle);
+                 the same figure is given for the two comparison paths
   latency_ms     p50 / p99 of the per-step times
   roofline       dominant kernel: achieved vs peak, measured with HIP events on its stream
   cpu_baseline   the oracle (C restatement, float32 reference semantics) timed on this host's
@@ -36,6 +41,7 @@ from sigpde_amd.synthetic import brownian_paths  # noqa: E402
 MI355X_HBM_GBS = 8000.0
 MI355X_FP64_VALU_TFLOPS = 78.6
 MI355X_FP16_MFMA_TFLOPS = 2500.0
+MI355X_FP32_MFMA_TFLOPS = 157.3
 FLOPS_PER_CELL = 10  # pde_step as written
 
 
@@ -44,8 +50,9 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--mode", default="fp16x2", choices=["fp16x2", "ref"])
-    p.add_argument("--ref-steps", type=int, default=1, help="timed steps of the plain FP32 drop-in")
+    p.add_argument("--mode", default="fp16x2", choices=["fp16x2", "fp32", "ref"])
+    p.add_argument("--ref-steps", type=int, default=1,
+                   help="timed steps of the reference-scheme FP32 path")
     p.add_argument("--paths", type=int, default=1024)
     p.add_argument("--length", type=int, default=256)
     p.add_argument("--dim", type=int, default=64)
@@ -84,11 +91,11 @@ def fused_roofline(args, X, Y, ms):
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     reps = 2
-    gram_step(args, X, Y, "fp16x2")
+    gram_step(args, X, Y, args.mode)
     torch.cuda.synchronize()
     e0.record(st)
     for _ in range(reps):
-        gram_step(args, X, Y, "fp16x2")
+        gram_step(args, X, Y, args.mode)
     e1.record(st)
     torch.cuda.synchronize()
     dur_s = e0.elapsed_time(e1) / 1e3 / reps
@@ -98,8 +105,12 @@ def fused_roofline(args, X, Y, ms):
     cells = pairs * m * n
     flops_wave = cells * FLOPS_PER_CELL
     flops_inc = pairs * 2 * args.dim * (X.shape[1] - 1) * (Y.shape[1] - 1)
+    name = {"fp16x2": "k_linear_gram_fused<float, 2, 1> (fp16x2 MFMA increments + double "
+                      "wavefront, one launch)",
+            "fp32": "k_linear_gram_fused<float, 2, 2> (fp32 MFMA increments + float32-typed "
+                    "wavefront, one launch)"}[args.mode]
     return {
-        "kernel": "k_linear_gram_mixed (fp16x2 MFMA increments + double wavefront, one launch)",
+        "kernel": name,
         "bound": "valu",
         "achieved": flops_wave / dur_s / 1e12,
         "peak": MI355X_FP64_VALU_TFLOPS,
@@ -109,7 +120,8 @@ def fused_roofline(args, X, Y, ms):
         "launch_ms": dur_s * 1e3,
         "cells_per_s": cells / dur_s,
         "increment_gram_tflops": flops_inc / dur_s / 1e12,
-        "increment_gram_mfma_frac": flops_inc / dur_s / 1e12 / MI355X_FP16_MFMA_TFLOPS,
+        "increment_gram_mfma_frac": flops_inc / dur_s / 1e12 / (
+            MI355X_FP16_MFMA_TFLOPS if args.mode == "fp16x2" else MI355X_FP32_MFMA_TFLOPS),
         "note": "wavefront flops = 10 per refined cell (pde_step as written); the increment Gram "
                 "(2*d flops per increment) runs on MFMA beside it",
     }
@@ -157,7 +169,7 @@ def dominant_kernel_roofline(args, X, Y):
     }
 
 
-def cpu_leg(args, X, Y, K, Kref):
+def cpu_leg(args, X, Y, K, others):
     """Runs the oracle: (1) the float64 reference on a sampled block for the accuracy figures,
     (2) the float32-semantics restatement timed on this host's cores as the CPU baseline."""
     import oracle
@@ -169,8 +181,10 @@ def cpu_leg(args, X, Y, K, Kref):
     Kg = K[:s, :s].double().cpu().numpy()
     rel = float(np.linalg.norm(Kg - K64) / np.linalg.norm(K64))
     per = np.abs(Kg - K64) / np.abs(K64)
-    Kr = Kref[:s, :s].double().cpu().numpy()
-    rel_ref = float(np.linalg.norm(Kr - K64) / np.linalg.norm(K64))
+    rel_others = {}
+    for name, Ko in others.items():
+        Kr = Ko[:s, :s].double().cpu().numpy()
+        rel_others[name] = float(np.linalg.norm(Kr - K64) / np.linalg.norm(K64))
 
     cores = min(os.cpu_count() or 1, 16)
     oracle.set_num_threads(cores)
@@ -191,8 +205,7 @@ def cpu_leg(args, X, Y, K, Kref):
         "rel_err_fp64_max": float(per.max()),
         "rel_err_fp64_p99": float(np.percentile(per, 99)),
         "rel_err_sample": f"{s}x{s} pairs",
-        "plain_fp32_rel_err_fp64": rel_ref,
-    }, {
+    }, rel_others, {
         "value": side2 * side2 / dt2,
         "unit": "pairs/s",
         "cores": cores,
@@ -215,12 +228,14 @@ def main():
 
     K, ms, step_ms = timed(args, X, Y, args.mode, args.steps, args.warmup)
     pairs = X.shape[0] * Y.shape[0]
-    # the plain FP32 drop-in, same workload, for the "vs plain FP32" comparison
+    # comparison paths on the same workload
+    K32, ms32, _ = timed(args, X, Y, "fp32", args.steps, 1) if args.mode != "fp32" else (K, ms, step_ms)
     Kref, ms_ref, _ = timed(args, X, Y, "ref", args.ref_steps, 0) if args.mode != "ref" else (K, ms, step_ms)
 
-    roof = fused_roofline(args, X, Y, ms) if args.mode == "fp16x2" else dominant_kernel_roofline(args, X, Y)
+    roof = (fused_roofline(args, X, Y, ms) if args.mode != "ref"
+            else dominant_kernel_roofline(args, X, Y))
     line = {
-        "metric": "Gram pairs/sec (mixed-precision fp16x2 path; plain FP32 drop-in beside it) at "
+        "metric": "Gram pairs/sec (mixed-precision fp16x2 path; plain FP32 beside it) at "
                   "rel err vs FP64 oracle",
         "value": pairs / (ms / 1e3),
         "unit": "pairs/s",
@@ -231,7 +246,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp16x2+f64" if args.mode == "fp16x2" else "f32",
+        "dtype": {"fp16x2": "fp16x2+f64", "fp32": "f32", "ref": "f32"}[args.mode],
         "data": "synthetic",
         "config": {"workload": f"SigPDE(LinearKernel(), {args.order}).gram(X, Y), X,Y "
                                f"{args.paths} Brownian paths, length {args.length}, dim "
@@ -240,13 +255,21 @@ def main():
                    "dim": args.dim, "dyadic_order": args.order},
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
                        "p99": float(np.percentile(step_ms, 99))},
-        "plain_fp32": {"value": pairs / (ms_ref / 1e3), "unit": "pairs/s", "ms_per_step": ms_ref},
-        "speedup_vs_plain_fp32": ms_ref / ms,
+        "plain_fp32": {"value": pairs / (ms32 / 1e3), "unit": "pairs/s", "ms_per_step": ms32,
+                       "path": "precision='fp32': fused, fp32 MFMA increments, float32 wavefront"},
+        "speedup_vs_plain_fp32": ms32 / ms,
+        "reference_scheme_fp32": {"value": pairs / (ms_ref / 1e3), "unit": "pairs/s",
+                                  "ms_per_step": ms_ref,
+                                  "path": "precision='ref': torch increment tiles (point Gram, "
+                                          "4-term difference) + generic wavefront"},
         "roofline": roof,
     }
     if not args.no_cpu:
-        acc, cpu = cpu_leg(args, X, Y, K, Kref)
+        acc, rel_others, cpu = cpu_leg(args, X, Y, K, {"plain_fp32": K32,
+                                                        "reference_scheme_fp32": Kref})
         line.update(acc)
+        for name, rel in rel_others.items():
+            line[name]["rel_err_fp64"] = rel
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
 

@@ -99,15 +99,19 @@ int sigpde_robust_norm(int dtype, const void *inc, int64_t B, int64_t m, int64_t
  * 145-167) for the linear kernel.
  *   precision SIGPDE_PREC_FP16X2: increments split into per-path-scaled fp16 hi + lo, products
  *     accumulated in fp32 on MFMA; the wavefront runs in double; results in dtype.
+ *   precision SIGPDE_PREC_FP32: plain FP32 -- increments rounded to float, products on fp32
+ *     MFMA, the wavefront in the reference's float32 typing (c, c^2 in float, the rest in double,
+ *     cells stored as float); results in dtype.
  *   X : (Bx, Lx, d), Y : (By, Ly, d) paths of dtype, C-contiguous; 1 <= d <= 128
  *   sx (Bx,), sy (By,) : optional scales of dtype (c *= sx[i] * sy[j]), or NULL
  *   gram: out[i * ldo + j] = K(x_i, y_j); symmetric != 0 requires Y == X and writes the lower
  *     triangle and its mirror.
  *   pairwise: out[b] = K(x_b, y_b) for Bx == By == B.
- *   workspace : device buffer of at least sigpde_linear_workspace_bytes(...) bytes (fp16 hi/lo
- *     increments and per-path scales); same_paths != 0 when Y is X (one split serves both).
+ *   workspace : device buffer of at least sigpde_linear_workspace_bytes(precision, ...) bytes
+ *     (the device-side increments and per-path scales); same_paths != 0 when Y is X (one split
+ *     serves both). Returns 0 for invalid sizes or an unknown precision.
  */
-enum sigpde_precision { SIGPDE_PREC_FP16X2 = 1 };
+enum sigpde_precision { SIGPDE_PREC_FP16X2 = 1, SIGPDE_PREC_FP32 = 2 };
 
 size_t sigpde_linear_workspace_bytes(int precision, int64_t Bx, int64_t Lx, int64_t By, int64_t Ly,
                                      int64_t d, int same_paths);

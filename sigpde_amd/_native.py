@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(_HERE, "libsigpde_hip.so")
 F32, F64 = 0, 1
 NORM_LOG, NORM_GIVEN = 0, 1
 PREC_FP16X2 = 1
-PRECISIONS = {"fp16x2": PREC_FP16X2}
+PREC_FP32 = 2
+PRECISIONS = {"fp16x2": PREC_FP16X2, "fp32": PREC_FP32}
 
 _lib = None
 _lock = threading.Lock()

@@ -10,6 +10,10 @@
 // v_mfma_f32_16x16x32_f16 and go to the ring unscaled; the wavefront runs in double and applies
 // the pair's scale m = kscale 4^-order 2^-(ex+ey) (x_scale y_scale) through its coefficients
 // (Arith::Mixed). Relative error against the float64 reference: ~1e-6 on Brownian paths.
+//
+// Precision "fp32" (the plain FP32 fused path): increments rounded to float, products on
+// v_mfma_f32_16x16x4f32 in fp32, scaled in fp32, and the wavefront in the reference's float32
+// typing (Arith::Ref32: c and c^2 in float, the rest in double, cells stored as float).
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -61,13 +65,31 @@ __global__ __launch_bounds__(256) void k_split_fp16(const T *__restrict__ X, int
   }
 }
 
+// fp32 operands: one block per path, dx = x[s+1] - x[s] in the input type, rounded to float,
+// rows of dpad floats (zero padded).
+template <typename T>
+__global__ __launch_bounds__(256) void k_split_f32(const T *__restrict__ X, int L, int d, int dpad,
+                                                   float *__restrict__ out) {
+  const int64_t b = blockIdx.x;
+  const T *x = X + b * (int64_t)L * d;
+  const int m = L - 1;
+  float *o = out + b * (int64_t)m * dpad;
+  for (int i = threadIdx.x; i < m * dpad; i += blockDim.x) {
+    const int s = i / dpad, k = i - s * dpad;
+    o[i] = k < d ? (float)(x[(s + 1) * d + k] - x[s * d + k]) : 0.0f;
+  }
+}
+
 // ------------------------------------------------------------------ kernels
 // Gram: a wave takes four x paths (a quad) and streams `ys` y paths through them.
 // Symmetric: only y <= x pairs are written, with their mirror.
 
-template <typename TO, int KB>
-__global__ __launch_bounds__(64) void k_linear_gram_mixed(GramFusedArgs<TO> a,
+template <typename TO, int KB, int PREC>
+__global__ __launch_bounds__(64) void k_linear_gram_fused(GramFusedArgs<TO> a,
                                                           const double *sxd, const double *syd) {
+  using Prod = MfmaProducer<KB, true, PREC>;
+  using E = typename Prod::elt;
+  constexpr Arith A = PREC == kPrecFp16x2 ? Arith::Mixed : Arith::Ref32;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int G = 16;
   const int lane = threadIdx.x & 63;
@@ -90,17 +112,19 @@ __global__ __launch_bounds__(64) void k_linear_gram_mixed(GramFusedArgs<TO> a,
     if (y0 + ys - 1 > xmax) ys = (int)(xmax - y0 + 1);
   }
   PairGeom<G> geo(a.m, a.n, a.order, ys);
-  MfmaProducer<KB, true> P;
-  P.xh = a.xh;
-  P.xl = a.xl;
-  P.yh = a.yh;
-  P.yl = a.yl;
+  Prod P;
+  P.xh = (const E *)a.xh;
+  P.xl = (const E *)a.xl;
+  P.yh = (const E *)a.yh;
+  P.yl = (const E *)a.yl;
   P.dpad = a.dpad;
   P.m = a.m;
   P.n = a.n;
   P.ystep = 1;
   P.invy = a.invy;
+  P.sx = sxd;
   P.sy = syd;
+  P.kmult = a.kmult;
   P.ny = a.By;
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
@@ -111,11 +135,11 @@ __global__ __launch_bounds__(64) void k_linear_gram_mixed(GramFusedArgs<TO> a,
   const int64_t xme = xq * 4 + lane / G;
   {
     const int64_t xc = xme < a.Bx ? xme : a.Bx - 1;
-    P.xsl = a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0);
+    P.xsl = PREC == kPrecFp16x2 ? a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0) : 1.0;
     P.ysl = 1.0;
     P.pmY = 1.0;
   }
-  ring_solve<Arith::Mixed, G>(P, geo, 1.0, smem_raw, [&](int sp, double r) {
+  ring_solve<A, G>(P, geo, 1.0, smem_raw, [&](int sp, double r) {
     const int64_t y = y0 + sp;
     if (xme >= a.Bx || y >= a.By) return;
     if (a.symmetric) {
@@ -137,8 +161,8 @@ __global__ void k_to_double(const T *src, int64_t n, double *dst) {
 
 // ------------------------------------------------------------------ host side
 struct Workspace {
-  _Float16 *xh, *xl, *yh, *yl;
-  float *invx, *invy;
+  void *xh, *xl, *yh, *yl;  // fp16x2: hi / lo halves; fp32: floats in xh / yh (xl = yl = null)
+  float *invx, *invy;       // fp16x2 only
   double *sxd, *syd;
 };
 
@@ -146,8 +170,8 @@ static inline size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
 static int dpad_for(int64_t d) { return (int)((d + kKBlock - 1) / kKBlock * kKBlock); }
 
-static size_t workspace_layout(int64_t Bx, int64_t Lx, int64_t By, int64_t Ly, int64_t d,
-                               int same, char *base, Workspace *w) {
+static size_t workspace_layout(int precision, int64_t Bx, int64_t Lx, int64_t By, int64_t Ly,
+                               int64_t d, int same, char *base, Workspace *w) {
   const size_t dp = (size_t)dpad_for(d);
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -156,24 +180,33 @@ static size_t workspace_layout(int64_t Bx, int64_t Lx, int64_t By, int64_t Ly, i
     return p;
   };
   const size_t nx = (size_t)Bx * (Lx - 1) * dp, ny = (size_t)By * (Ly - 1) * dp;
-  Workspace t;
-  t.xh = (_Float16 *)take(nx * 2);
-  t.xl = (_Float16 *)take(nx * 2);
-  t.yh = same ? t.xh : (_Float16 *)take(ny * 2);
-  t.yl = same ? t.xl : (_Float16 *)take(ny * 2);
-  t.invx = (float *)take(Bx * 4);
-  t.invy = same ? t.invx : (float *)take(By * 4);
+  Workspace t{};
+  if (precision == SIGPDE_PREC_FP32) {
+    t.xh = take(nx * 4);
+    t.yh = same ? t.xh : take(ny * 4);
+  } else {
+    t.xh = take(nx * 2);
+    t.xl = take(nx * 2);
+    t.yh = same ? t.xh : take(ny * 2);
+    t.yl = same ? t.xl : take(ny * 2);
+    t.invx = (float *)take(Bx * 4);
+    t.invy = same ? t.invx : (float *)take(By * 4);
+  }
   t.sxd = (double *)take(Bx * 8);
   t.syd = (double *)take(By * 8);
   if (w) *w = t;
   return off;
 }
 
-template <typename T>
-static hipError_t run_split(const T *X, int64_t B, int64_t L, int64_t d, int dp, _Float16 *h,
-                            _Float16 *l, float *inv, hipStream_t st) {
+template <typename T, int PREC>
+static hipError_t run_split(const T *X, int64_t B, int64_t L, int64_t d, int dp, void *h, void *l,
+                            float *inv, hipStream_t st) {
   if (B == 0) return hipSuccess;
-  k_split_fp16<T><<<dim3((unsigned)B), dim3(256), 0, st>>>(X, B, (int)L, (int)d, dp, h, l, inv);
+  if (PREC == kPrecFp32)
+    k_split_f32<T><<<dim3((unsigned)B), dim3(256), 0, st>>>(X, (int)L, (int)d, dp, (float *)h);
+  else
+    k_split_fp16<T><<<dim3((unsigned)B), dim3(256), 0, st>>>(X, B, (int)L, (int)d, dp,
+                                                             (_Float16 *)h, (_Float16 *)l, inv);
   return hipGetLastError();
 }
 
@@ -184,16 +217,16 @@ static hipError_t run_to_double(const void *src, int64_t n, double *dst, hipStre
   return hipGetLastError();
 }
 
-template <typename T, int KB>
+template <typename T, int KB, int PREC>
 static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int64_t Bx, int64_t By,
                                int64_t Lx, int64_t Ly, int64_t d, double kscale, int order,
                                const void *sx, const void *sy, int symmetric, void *out,
                                int64_t ldo, const Workspace &w, hipStream_t st) {
   const int dp = dpad_for(d);
-  hipError_t e = run_split<T>((const T *)X, Bx, Lx, d, dp, w.xh, w.xl, w.invx, st);
+  hipError_t e = run_split<T, PREC>((const T *)X, Bx, Lx, d, dp, w.xh, w.xl, w.invx, st);
   if (e != hipSuccess) return e;
   if (w.yh != w.xh) {
-    e = run_split<T>((const T *)Y, By, Ly, d, dp, w.yh, w.yl, w.invy, st);
+    e = run_split<T, PREC>((const T *)Y, By, Ly, d, dp, w.yh, w.yl, w.invy, st);
     if (e != hipSuccess) return e;
   }
   if ((e = run_to_double<T>(sx, Bx, w.sxd, st)) != hipSuccess) return e;
@@ -217,9 +250,9 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
   a.ldo = ldo;
   const double *sxd = sx ? w.sxd : nullptr;
   const double *syd = sy ? w.syd : nullptr;
-  const size_t shmem = wave_lds_bytes<Arith::Mixed, 16>((int)((Ly - 1) << order));
+  const size_t shmem = wave_lds_bytes<Arith::Mixed, 16>((int)((Ly - 1) << order));  // = Ref32's
   if (pairwise) {
-    return launch_linear_pairwise_mixed<T, KB>(a, sxd, syd, shmem, st);
+    return launch_linear_pairwise_fused<T, KB, PREC>(a, sxd, syd, shmem, st);
   } else {
     // y paths streamed per wave: enough waves to fill the GPU, long enough streams to amortise
     // the 15-step pipeline fill of each pair
@@ -230,33 +263,43 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
     const int64_t nblk = nq * ((By + ys - 1) / ys);
     // a multiple of 8 blocks so that the XCD-major block order is a bijection onto the work units
     const int64_t grid = (nblk + 7) / 8 * 8;
-    k_linear_gram_mixed<T, KB><<<dim3((unsigned)grid), dim3(64), shmem, st>>>(a, sxd, syd);
+    k_linear_gram_fused<T, KB, PREC><<<dim3((unsigned)grid), dim3(64), shmem, st>>>(a, sxd, syd);
   }
   return hipGetLastError();
 }
 
-template <typename T>
+template <typename T, int PREC>
 static hipError_t dispatch_kb(bool pairwise, const void *X, const void *Y, int64_t Bx, int64_t By,
                               int64_t Lx, int64_t Ly, int64_t d, double kscale, int order,
                               const void *sx, const void *sy, int symmetric, void *out,
                               int64_t ldo, const Workspace &w, hipStream_t st) {
   const int kb = dpad_for(d) / kKBlock;
+#define SIGPDE_KB_CASE(K)                                                                          \
+  case K:                                                                                          \
+    return launch_fused<T, K, PREC>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,       \
+                                    symmetric, out, ldo, w, st);
   switch (kb) {
-    case 1:
-      return launch_fused<T, 1>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
-                                symmetric, out, ldo, w, st);
-    case 2:
-      return launch_fused<T, 2>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
-                                symmetric, out, ldo, w, st);
-    case 3:
-      return launch_fused<T, 3>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
-                                symmetric, out, ldo, w, st);
-    case 4:
-      return launch_fused<T, 4>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
-                                symmetric, out, ldo, w, st);
+    SIGPDE_KB_CASE(1)
+    SIGPDE_KB_CASE(2)
+    SIGPDE_KB_CASE(3)
+    SIGPDE_KB_CASE(4)
     default:
       return hipErrorInvalidValue;
   }
+#undef SIGPDE_KB_CASE
+}
+
+template <typename T>
+static hipError_t dispatch_prec(int precision, bool pairwise, const void *X, const void *Y,
+                                int64_t Bx, int64_t By, int64_t Lx, int64_t Ly, int64_t d,
+                                double kscale, int order, const void *sx, const void *sy,
+                                int symmetric, void *out, int64_t ldo, const Workspace &w,
+                                hipStream_t st) {
+  if (precision == SIGPDE_PREC_FP32)
+    return dispatch_kb<T, kPrecFp32>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
+                                     symmetric, out, ldo, w, st);
+  return dispatch_kb<T, kPrecFp16x2>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
+                                     symmetric, out, ldo, w, st);
 }
 
 constexpr int64_t kMaxDim = 4 * kKBlock;
@@ -266,7 +309,7 @@ static int check_fused(int dtype, int precision, const void *X, const void *Y, i
                        void *ws) {
   if (dtype != SIGPDE_F32 && dtype != SIGPDE_F64)
     return set_error(SIGPDE_EINVAL, "linear: unknown dtype %d", dtype);
-  if (precision != SIGPDE_PREC_FP16X2)
+  if (precision != SIGPDE_PREC_FP16X2 && precision != SIGPDE_PREC_FP32)
     return set_error(SIGPDE_EINVAL, "linear: unsupported precision mode %d", precision);
   if (Bx < 0 || By < 0 || !X || !Y || !out || !ws)
     return set_error(SIGPDE_EINVAL, "linear: bad batch or null pointer");
@@ -284,9 +327,9 @@ using namespace sigpde;
 
 extern "C" size_t sigpde_linear_workspace_bytes(int precision, int64_t Bx, int64_t Lx, int64_t By,
                                                int64_t Ly, int64_t d, int same_paths) {
-  (void)precision;
   if (Bx < 0 || By < 0 || Lx < 2 || Ly < 2 || d < 1) return 0;
-  return workspace_layout(Bx, Lx, By, Ly, d, same_paths, nullptr, nullptr);
+  if (precision != SIGPDE_PREC_FP16X2 && precision != SIGPDE_PREC_FP32) return 0;
+  return workspace_layout(precision, Bx, Lx, By, Ly, d, same_paths, nullptr, nullptr);
 }
 
 extern "C" int sigpde_linear_gram(int dtype, int precision, const void *X, int64_t Bx, int64_t Lx,
@@ -302,14 +345,14 @@ extern "C" int sigpde_linear_gram(int dtype, int precision, const void *X, int64
     return set_error(SIGPDE_EINVAL, "linear_gram: symmetric needs Y == X");
   const int same = X == Y && Bx == By && Lx == Ly;
   Workspace w;
-  const size_t need = workspace_layout(Bx, Lx, By, Ly, d, same, (char *)workspace, &w);
+  const size_t need = workspace_layout(precision, Bx, Lx, By, Ly, d, same, (char *)workspace, &w);
   if (ws_bytes < need)
     return set_error(SIGPDE_EINVAL, "linear_gram: workspace of %zu bytes, need %zu", ws_bytes, need);
   hipStream_t st = (hipStream_t)stream;
   hipError_t e = dtype == SIGPDE_F32
-                     ? dispatch_kb<float>(false, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
+                     ? dispatch_prec<float>(precision, false, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
                                           symmetric, out, ldo, w, st)
-                     : dispatch_kb<double>(false, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
+                     : dispatch_prec<double>(precision, false, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
                                            symmetric, out, ldo, w, st);
   return hip_status(e, "linear_gram launch");
 }
@@ -324,14 +367,14 @@ extern "C" int sigpde_linear_pairwise(int dtype, int precision, const void *X, c
   if (B == 0) return SIGPDE_OK;
   const int same = X == Y && Lx == Ly;
   Workspace w;
-  const size_t need = workspace_layout(B, Lx, B, Ly, d, same, (char *)workspace, &w);
+  const size_t need = workspace_layout(precision, B, Lx, B, Ly, d, same, (char *)workspace, &w);
   if (ws_bytes < need)
     return set_error(SIGPDE_EINVAL, "linear_pairwise: workspace of %zu bytes, need %zu", ws_bytes, need);
   hipStream_t st = (hipStream_t)stream;
   hipError_t e = dtype == SIGPDE_F32
-                     ? dispatch_kb<float>(true, X, Y, B, B, Lx, Ly, d, kscale, order, sx, sy, 0,
+                     ? dispatch_prec<float>(precision, true, X, Y, B, B, Lx, Ly, d, kscale, order, sx, sy, 0,
                                           out, 1, w, st)
-                     : dispatch_kb<double>(true, X, Y, B, B, Lx, Ly, d, kscale, order, sx, sy, 0,
+                     : dispatch_prec<double>(precision, true, X, Y, B, B, Lx, Ly, d, kscale, order, sx, sy, 0,
                                            out, 1, w, st);
   return hip_status(e, "linear_pairwise launch");
 }

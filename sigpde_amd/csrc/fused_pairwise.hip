@@ -1,31 +1,36 @@
-// Fused linear-kernel pairwise solve (precision "fp16x2"): four pairs (x_b, y_b) per wave, the
-// increment Gram tiles produced on MFMA into the wavefront's LDS ring. Same arithmetic as the
-// Gram kernel in fused_linear.hip (reference: sigpde/torch/kernel.py:13-16 with
+// Fused linear-kernel pairwise solve (precisions "fp16x2" and "fp32"): four pairs (x_b, y_b) per
+// wave, the increment Gram tiles produced on MFMA into the wavefront's LDS ring. Same arithmetic
+// as the Gram kernel in fused_linear.hip (reference: sigpde/torch/kernel.py:13-16 with
 // sigpde/torch/static_kernels.py:7-8, solved as sigpde/cuda_pairwise_kernels.py:8-85).
 #include "mfma_producer.h"
 
 namespace sigpde {
 
 // Pairwise: a wave solves four pairs (x_b, y_b).
-template <typename TO, int KB>
-__global__ __launch_bounds__(64) void k_linear_pairwise_mixed(GramFusedArgs<TO> a,
+template <typename TO, int KB, int PREC>
+__global__ __launch_bounds__(64) void k_linear_pairwise_fused(GramFusedArgs<TO> a,
                                                               const double *sxd,
                                                               const double *syd) {
+  using Prod = MfmaProducer<KB, false, PREC>;
+  using E = typename Prod::elt;
+  constexpr Arith A = PREC == kPrecFp16x2 ? Arith::Mixed : Arith::Ref32;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int G = 16;
   const int lane = threadIdx.x & 63;
   PairGeom<G> geo(a.m, a.n, a.order, 1);
-  MfmaProducer<KB, false> P;
-  P.xh = a.xh;
-  P.xl = a.xl;
-  P.yh = a.yh;
-  P.yl = a.yl;
+  Prod P;
+  P.xh = (const E *)a.xh;
+  P.xl = (const E *)a.xl;
+  P.yh = (const E *)a.yh;
+  P.yl = (const E *)a.yl;
   P.dpad = a.dpad;
   P.m = a.m;
   P.n = a.n;
   P.ystep = 0;
   P.invy = a.invy;
+  P.sx = sxd;
   P.sy = syd;
+  P.kmult = a.kmult;
   P.ny = a.By;
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
@@ -37,27 +42,32 @@ __global__ __launch_bounds__(64) void k_linear_pairwise_mixed(GramFusedArgs<TO> 
   {
     const int64_t xc = bme < a.Bx ? bme : a.Bx - 1;
     const int64_t yc = bme < a.By ? bme : a.By - 1;
-    P.xsl = a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0);
-    P.ysl = (double)a.invy[yc] * (syd ? syd[yc] : 1.0);
+    if (PREC == kPrecFp16x2) {
+      P.xsl = a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0);
+      P.ysl = (double)a.invy[yc] * (syd ? syd[yc] : 1.0);
+    } else {
+      P.xsl = P.ysl = 1.0;
+    }
     P.pmY = 1.0;
   }
-  ring_solve<Arith::Mixed, G>(P, geo, 1.0, smem_raw, [&](int, double r) {
+  ring_solve<A, G>(P, geo, 1.0, smem_raw, [&](int, double r) {
     if (bme < a.Bx) a.out[bme] = (TO)r;
   });
 }
 
-template <typename T, int KB>
-hipError_t launch_linear_pairwise_mixed(const GramFusedArgs<T> &a, const double *sxd,
+template <typename T, int KB, int PREC>
+hipError_t launch_linear_pairwise_fused(const GramFusedArgs<T> &a, const double *sxd,
                                         const double *syd, size_t shmem, hipStream_t st) {
   const unsigned blocks = (unsigned)((a.Bx + 3) / 4);
-  k_linear_pairwise_mixed<T, KB><<<dim3(blocks), dim3(64), shmem, st>>>(a, sxd, syd);
+  k_linear_pairwise_fused<T, KB, PREC><<<dim3(blocks), dim3(64), shmem, st>>>(a, sxd, syd);
   return hipGetLastError();
 }
 
 #define SIGPDE_PAIRWISE_INST(T, KB)                                                               \
-  template hipError_t launch_linear_pairwise_mixed<T, KB>(const GramFusedArgs<T> &,              \
-                                                          const double *, const double *, size_t, \
-                                                          hipStream_t);
+  template hipError_t launch_linear_pairwise_fused<T, KB, kPrecFp16x2>(                          \
+      const GramFusedArgs<T> &, const double *, const double *, size_t, hipStream_t);          \
+  template hipError_t launch_linear_pairwise_fused<T, KB, kPrecFp32>(                            \
+      const GramFusedArgs<T> &, const double *, const double *, size_t, hipStream_t);
 SIGPDE_PAIRWISE_INST(float, 1)
 SIGPDE_PAIRWISE_INST(float, 2)
 SIGPDE_PAIRWISE_INST(float, 3)

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "ring_solver.h"
 
 namespace sigpde {
@@ -12,7 +14,11 @@ namespace sigpde {
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 
-constexpr int kKBlock = 32;  // K of one v_mfma_f32_16x16x32_f16
+constexpr int kKBlock = 32;  // dimension padding unit (K of one v_mfma_f32_16x16x32_f16)
+
+// Operand precisions of the fused producer (values of SIGPDE_PREC_*).
+constexpr int kPrecFp16x2 = 1;  // per-path-scaled fp16 hi + lo, 3 f16 MFMAs per 32-wide K block
+constexpr int kPrecFp32 = 2;    // fp32 increments, v_mfma_f32_16x16x4f32
 
 // ------------------------------------------------------------------ MFMA producer (G = 16)
 // Lane group g (16 lanes) solves pairs (x path xa[g], y path ybase[g] + sp * ystep) of the stream.
@@ -21,32 +27,44 @@ constexpr int kKBlock = 32;  // K of one v_mfma_f32_16x16x32_f16
 // B = the chunk's y-increment columns (refined indices repeat original rows / columns 2^order
 // times; columns outside 1..N read a clamped column, see ring_solver.h). Lane l holds
 // D[4(l/16) + e][l % 16] and writes those four ring rows of column l % 16.
+//   fp16x2: lane l supplies K block kb's elements 8(l/16)..+7 (one half8 per block, hi and lo);
+//           the tiles go to the ring unscaled (the consumer applies the pair's scale, Mixed).
+//   fp32:   lane l supplies elements (l/16)*dpad/4 + j, j < dpad/4, one per MFMA (any split of
+//           K works as long as A and B use the same one); the tiles are scaled by
+//           kscale/4^order (x_scale y_scale) in fp32 and the consumer runs Ref32.
 // Pipeline of period p: at its start, multiply chunk p+1 and fetch chunk p+2's B operands; before
 // its last consumer group, write chunk p+1's tiles (the accumulators never live across periods).
 // After the last chunk of a band is multiplied, the next band's A operands are loaded. Operand
 // rows are addressed as a wave-uniform 64-bit row base plus a 32-bit per-lane offset.
-template <int KB, bool kSharedY>
+template <int KB, bool kSharedY, int PREC>
 struct MfmaProducer {
   static constexpr int G = 16;
   static constexpr int NP = 4;           // pairs (lane groups) per wave
   static constexpr int NB = kSharedY ? 1 : NP;
-  const _Float16 *xh, *xl, *yh, *yl;
+  static constexpr bool kF16 = PREC == kPrecFp16x2;
+  static constexpr int NV = kF16 ? KB : 2 * KB;  // operand vectors per lane (half8 or float4)
+  typedef typename std::conditional<kF16, half8, float4v>::type vec;
+  typedef typename std::conditional<kF16, _Float16, float>::type elt;
+  const elt *xh, *xl, *yh, *yl;          // fp32: xh / yh only
   int dpad;
   int m, n;                              // original increments per x / y path
   int64_t xa[NP];                        // x path of each group (-1: idle)
   int64_t ybase[NP];
   int ystep;
   const float *invy;
-  const double *sy;                      // optional y scales (robust Gram), may be null
+  const double *sx, *sy;                 // optional path scales (robust Gram), may be null
   int64_t ny;                            // valid y paths (ybase + sp*ystep < ny)
-  double xsl;                            // per lane: kscale 4^-order 2^-ex (x_scale) of its x path
-  double ysl;                            // per lane (pairwise): 2^-ey (y_scale) of its y path
-  double pmY;                            // Gram: 2^-ey (y_scale) of the pair multiplied last
+  double kmult;                          // fp32: kscale / 4^order
+  double xsl;                            // fp16x2, per lane: kscale 4^-order 2^-ex (x_scale)
+  double ysl;                            // fp16x2 pairwise, per lane: 2^-ey (y_scale)
+  double pmY;                            // fp16x2 Gram: 2^-ey (y_scale) of the pair multiplied last
 
-  half8 Ah[NP][KB], Al[NP][KB];          // A operands of the band being multiplied
-  half8 Bh[NB][KB], Bl[NB][KB];          // next chunk's B operands (fetched one period ahead)
+  vec Ah[NP][NV], Al[NP][kF16 ? NV : 1];   // A operands of the band being multiplied
+  vec Bh[NB][NV], Bl[NB][kF16 ? NV : 1];   // next chunk's B operands (fetched one period ahead)
   float4v acc[NP];
-  int64_t yrow[NB];                      // first B row (in halves) of the pair being fetched
+  float mult[NP];                        // fp32: scale of the tiles in acc
+  float pm[NP];                          // fp32: scale of the pair being multiplied
+  int64_t yrow[NB];                      // first B row (in elements) of the pair being fetched
   Cursor<G> cB, cC;                      // next chunk to fetch B for / to multiply
   int slotS;                             // ring slot of the next chunk to store
 
@@ -56,21 +74,33 @@ struct MfmaProducer {
     const int64_t yp = ybase[g] + (int64_t)sp * ystep;
     return yp < ny ? yp : ny - 1;
   }
+  // per-lane element offset of this lane's K slice in an operand row
+  __device__ __forceinline__ uint32_t kslice() const {
+    const int h = (threadIdx.x & 63) >> 4;
+    return kF16 ? 8 * h : h * (dpad / 4);
+  }
+  __device__ __forceinline__ void load_row(const elt *ph, const elt *pl, uint32_t lo, vec *dh,
+                                           vec *dl) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      if constexpr (kF16) {
+        dh[v] = *reinterpret_cast<const vec *>(ph + lo + v * kKBlock);
+        dl[v] = *reinterpret_cast<const vec *>(pl + lo + v * kKBlock);
+      } else {
+        dh[v] = *reinterpret_cast<const vec *>(ph + lo + 4 * v);
+      }
+    }
+  }
 
   __device__ __forceinline__ void fetchA(const PairGeom<G> &geo, int band) {
     const int lane = threadIdx.x & 63;
     int ro = (band * G + (lane & 15)) >> geo.order;
     ro = ro > m - 1 ? m - 1 : ro;
-    const uint32_t lo = (uint32_t)(ro * dpad + 8 * (lane >> 4));
+    const uint32_t lo = (uint32_t)(ro * dpad) + kslice();
 #pragma unroll
     for (int g = 0; g < NP; ++g) {
       const int64_t xrow = (xa[g] < 0 ? 0 : xa[g]) * m * (int64_t)dpad;
-      const _Float16 *ph = xh + xrow, *pl = xl + xrow;
-#pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
-        Ah[g][kb] = *reinterpret_cast<const half8 *>(ph + lo + kb * kKBlock);
-        Al[g][kb] = *reinterpret_cast<const half8 *>(pl + lo + kb * kKBlock);
-      }
+      load_row(xh + xrow, kF16 ? xl + xrow : nullptr, lo, Ah[g], Al[g]);
     }
   }
   // B operands of chunk cB, then advance cB.
@@ -82,38 +112,55 @@ struct MfmaProducer {
     }
     int co = (cB.jv0 + (lane & 15) - 1) >> geo.order;
     co = co < 0 ? 0 : (co > n - 1 ? n - 1 : co);
-    const uint32_t lo = (uint32_t)(co * dpad + 8 * (lane >> 4));
+    const uint32_t lo = (uint32_t)(co * dpad) + kslice();
 #pragma unroll
-    for (int q = 0; q < NB; ++q) {
-      const _Float16 *ph = yh + yrow[q], *pl = yl + yrow[q];
-#pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
-        Bh[q][kb] = *reinterpret_cast<const half8 *>(ph + lo + kb * kKBlock);
-        Bl[q][kb] = *reinterpret_cast<const half8 *>(pl + lo + kb * kKBlock);
-      }
-    }
+    for (int q = 0; q < NB; ++q)
+      load_row(yh + yrow[q], kF16 ? yl + yrow[q] : nullptr, lo, Bh[q], Bl[q]);
     cB.advance(geo);
   }
   // Tiles of chunk cC, then advance cC; if the next chunk opens a band, load that band's A
   // operands now, into the registers this chunk has just finished reading (they land while the
   // wave computes).
   __device__ __forceinline__ void compute(const PairGeom<G> &geo, bool more) {
-    if (kSharedY && cC.band == 0 && cC.jv0 == 0) {
-      const int64_t yp = ypath(0, cC.sp);
-      pmY = sy ? (double)invy[yp] * sy[yp] : (double)invy[yp];
+    if (cC.band == 0 && cC.jv0 == 0) {
+      if constexpr (kF16) {
+        if (kSharedY) {
+          const int64_t yp = ypath(0, cC.sp);
+          pmY = sy ? (double)invy[yp] * sy[yp] : (double)invy[yp];
+        }
+      } else {
+#pragma unroll
+        for (int g = 0; g < NP; ++g) {
+          const int64_t xp = xa[g] < 0 ? 0 : xa[g];
+          const int64_t yp = ypath(g, cC.sp);
+          double mm = kmult;
+          if (sx) mm *= sx[xp];
+          if (sy) mm *= sy[yp];
+          pm[g] = (float)mm;
+        }
+      }
     }
 #pragma unroll
     for (int g = 0; g < NP; ++g) {
       const int q = kSharedY ? 0 : g;
       float4v a = {0.f, 0.f, 0.f, 0.f};
+      if constexpr (kF16) {
 #pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
-        a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al[g][kb], Bh[q][kb], a, 0, 0, 0);
-        a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[g][kb], Bl[q][kb], a, 0, 0, 0);
+        for (int kb = 0; kb < KB; ++kb) {
+          a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al[g][kb], Bh[q][kb], a, 0, 0, 0);
+          a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[g][kb], Bl[q][kb], a, 0, 0, 0);
+        }
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+          a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[g][kb], Bh[q][kb], a, 0, 0, 0);
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            a = __builtin_amdgcn_mfma_f32_16x16x4f32(Ah[g][v][e], Bh[q][v][e], a, 0, 0, 0);
+        mult[g] = pm[g];
       }
-#pragma unroll
-      for (int kb = 0; kb < KB; ++kb)
-        a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[g][kb], Bh[q][kb], a, 0, 0, 0);
       acc[g] = a;
     }
     cC.advance(geo);
@@ -126,8 +173,12 @@ struct MfmaProducer {
 #pragma unroll
     for (int g = 0; g < NP; ++g) {
       float *dst = ring + (slot0 + c) * kRingStride + g * G + 4 * (lane >> 4);
-      const float2 v0 = make_float2(acc[g][0], acc[g][1]);
-      const float2 v1 = make_float2(acc[g][2], acc[g][3]);
+      float2 v0 = make_float2(acc[g][0], acc[g][1]);
+      float2 v1 = make_float2(acc[g][2], acc[g][3]);
+      if constexpr (!kF16) {
+        v0 = make_float2(v0.x * mult[g], v0.y * mult[g]);
+        v1 = make_float2(v1.x * mult[g], v1.y * mult[g]);
+      }
       *reinterpret_cast<float2 *>(dst) = v0;
       *reinterpret_cast<float2 *>(dst + 2) = v1;
       if (slot0 == 0 && c < kRingMirror) {
@@ -167,8 +218,8 @@ struct MfmaProducer {
 // Arguments of the fused kernels (x / y operands from the split prepass).
 template <typename TO>
 struct GramFusedArgs {
-  const _Float16 *xh, *xl, *yh, *yl;
-  const float *invx, *invy;
+  const void *xh, *xl, *yh, *yl;  // fp16x2: _Float16 hi / lo rows; fp32: float rows in xh / yh
+  const float *invx, *invy;       // fp16x2: per-path 2^-e
   int64_t Bx, By;
   int m, n, order, dpad, ys;
   double kmult;  // kscale / 4^order
@@ -180,8 +231,8 @@ struct GramFusedArgs {
 // Pairwise launcher, defined in fused_pairwise.hip (a translation unit of its own: its four
 // independent B operand sets need the compiler's default accumulator placement, while the Gram
 // kernel is built with MFMA results in VGPRs).
-template <typename T, int KB>
-hipError_t launch_linear_pairwise_mixed(const GramFusedArgs<T> &a, const double *sxd,
+template <typename T, int KB, int PREC>
+hipError_t launch_linear_pairwise_fused(const GramFusedArgs<T> &a, const double *sxd,
                                         const double *syd, size_t shmem, hipStream_t st);
 
 }  // namespace sigpde

@@ -97,7 +97,7 @@ def _scale_vector(s, batch: int, like: torch.Tensor) -> torch.Tensor:
     return s.contiguous()
 
 
-PRECISIONS = ("ref", "fp16x2")
+PRECISIONS = ("ref", "fp16x2", "fp32")
 
 
 def _fused_scale(static_kernel, precision: str):
@@ -122,7 +122,10 @@ class SigPDE:
                   per-cell arithmetic for float32 inputs, all-double for float64 inputs;
       "fp16x2" -- linear kernel only: increments <dx, dy> formed from the paths and multiplied as
                   per-path-scaled fp16 hi/lo pairs on MFMA (fp32 accumulate), double wavefront;
-                  relative error ~1e-7 against the float64 reference scheme.
+                  relative error ~1e-6 against the float64 reference scheme;
+      "fp32"   -- linear kernel only, plain FP32 fused: increments rounded to float, <dx, dy> on
+                  fp32 MFMA, the wavefront in the reference's float32 typing (no increment
+                  tensor in HBM, as for "fp16x2").
     """
 
     def __init__(self, static_kernel, dyadic_order: int = 0, precision: str = "ref"):
