@@ -142,6 +142,13 @@ def robust_norm(inc: torch.Tensor, order: int, lam: torch.Tensor, knorm: torch.T
     _check(rc, "sigpde_robust_norm")
 
 
+def linear_workspace_bytes(precision: str, Bx: int, Lx: int, By: int, Ly: int, d: int,
+                           same: bool) -> int:
+    """Device workspace of the fused linear-kernel entry points (host-side size computation)."""
+    return int(load().sigpde_linear_workspace_bytes(PRECISIONS[precision], Bx, Lx, By, Ly, d,
+                                                     int(same)))
+
+
 def _workspace(nbytes: int, device) -> torch.Tensor:
     return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
 

@@ -103,8 +103,10 @@ struct ArithTraits<Arith::Scaled64> { using RT = double; static constexpr bool k
 template <>
 struct ArithTraits<Arith::Mixed> { using RT = float; static constexpr bool kRound = false; };
 
-// A = (1 + c/2) + c^2/12 and B = 1 - c^2/12 as pde_step writes them (the compiler may contract the
-// products into FMAs, as NVVM does for the reference).
+// A = (1 + c/2) + c^2/12 and B = 1 - c^2/12 as pde_step writes them, with the products contracted
+// into FMAs (as NVVM contracts the reference's expression). The contractions are written out so
+// that every kernel instantiation rounds a cell the same way: a pair's result does not depend on
+// which launch shape solved it.
 template <Arith A>
 __device__ __forceinline__ void coefs(typename ArithTraits<A>::RT c, double s, double h, double q,
                                       double &cA, double &cB) {
@@ -127,16 +129,16 @@ __device__ __forceinline__ void coefs(typename ArithTraits<A>::RT c, double s, d
     cd = (double)c;
     c2 = cd * cd;
   }
-  cA = (1.0 + 0.5 * cd) + (1.0 / 12) * c2;
-  cB = 1.0 - (1.0 / 12) * c2;
+  cA = fma(c2, 1.0 / 12, fma(cd, 0.5, 1.0));
+  cB = fma(c2, -1.0 / 12, 1.0);
 }
 
 // k = (k01 + k10) A - k00 B, grouped as k01*A + (k10*A - k00*B) so that only the last FMA waits
 // for the value shifted in from the row above.
 template <Arith A>
 __device__ __forceinline__ double cell(double k01, double k10, double k00, double cA, double cB) {
-  double w = k10 * cA - k00 * cB;
-  double r = k01 * cA + w;
+  const double w = fma(k10, cA, -(k00 * cB));
+  double r = fma(k01, cA, w);
   if constexpr (ArithTraits<A>::kRound) r = (double)(float)r;
   return r;
 }

@@ -1,0 +1,159 @@
+"""Serving loop on the GPU: a request's result is the same whether it is served alone (a direct API
+call) or inside a continuous-batching mix; sampled requests match the float64 oracle; the HBM
+budget is respected; HIP-graph replays equal eager launches."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import sigpde_amd.torch as sig
+from sigpde_amd import serving as S
+from helpers import brownian_np
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def quantized(B, L, d, seed):
+    # dyadic-grid paths: every point-Gram entry and increment is exact in float32, so results do
+    # not depend on the summation order of any product (bitwise comparisons are meaningful)
+    rng = np.random.default_rng(seed)
+    x = np.cumsum(rng.integers(-4, 5, size=(B, L, d)), axis=1) / 32.0
+    x[:, 0] = 0
+    return torch.from_numpy(x.astype(np.float32)).to(DEV)
+
+
+def alone(r):
+    k = r.static_kernel
+    if r.kind == "pairwise":
+        return sig.SigPDE(k, r.dyadic_order, r.precision).pairwise(r.x, r.y)
+    if r.kind == "gram":
+        return sig.SigPDE(k, r.dyadic_order, r.precision).gram(r.x, r.y)
+    rk = sig.RobustSigPDE(k, r.dyadic_order, r.normalizer)
+    if r.kind == "normalization":
+        return rk.normalization(r.x, tol=r.tol, maxit=r.maxit)
+    if r.kind == "robust_pairwise":
+        return rk.pairwise(r.x, r.y, tol=r.tol, maxit=r.maxit)
+    return rk.gram(r.x, r.y, tol=r.tol, maxit=r.maxit)
+
+
+def mix(seed, n=24):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n):
+        kind = S.KINDS[i % len(S.KINDS)]
+        B = int(rng.integers(1, 6))
+        L = int(rng.choice([9, 12, 17]))     # few lengths: several requests share a class
+        d = int(rng.choice([2, 4]))
+        order = int(rng.integers(0, 2))
+        x = quantized(B, L, d, 100 + i)
+        if kind in ("gram", "robust_gram"):
+            y = quantized(int(rng.integers(1, 5)), L, d, 200 + i) if i % 2 else None
+        elif kind == "normalization":
+            y = None
+        else:
+            y = quantized(B, L, d, 300 + i)
+        prec = "fp32" if (kind in ("pairwise", "gram") and i % 3 == 0) else "ref"
+        reqs.append(S.Request(kind, x, y, dyadic_order=order, precision=prec,
+                              tol=1e-6, maxit=60))
+    return reqs
+
+
+def test_alone_equals_in_mix_exact_lengths():
+    reqs = mix(1)
+    ref = [alone(r) for r in reqs]
+    srv = S.Server(DEV, bucket_lengths=None, streams=3, max_pairs=12)
+    srv.run(reqs, realtime=False)
+    torch.cuda.synchronize()
+    assert srv.launches < len(reqs)  # requests were actually batched together
+    for r, a in zip(reqs, ref):
+        assert r.error is None, r.error
+        assert torch.equal(r.result, a), (r.kind, r.id)
+
+
+def test_alone_equals_in_mix_with_length_buckets():
+    reqs = mix(2)
+    ref = [alone(r) for r in reqs]
+    srv = S.Server(DEV, streams=2)
+    srv.run(reqs, realtime=False)
+    torch.cuda.synchronize()
+    for r, a in zip(reqs, ref):
+        assert r.error is None, r.error
+        # padding adds zero-increment cells: equal up to float32 rounding of those cells
+        # (root-find results move within the root-find's tolerance)
+        tol = 1e-5 if r.kind in ("pairwise", "gram") else 2e-4
+        torch.testing.assert_close(r.result, a, rtol=tol, atol=1e-6)
+
+
+def test_sampled_requests_match_oracle():
+    rng = np.random.default_rng(3)
+    reqs = []
+    for i in range(6):
+        L = int(rng.choice([10, 14, 20]))
+        x = brownian_np(3, L, 3, 1 / L, 10 + i)
+        y = brownian_np(3, int(rng.choice([10, 14])), 3, 1 / L, 20 + i)
+        kind = "pairwise" if i % 2 else "gram"
+        reqs.append(S.Request(kind, torch.from_numpy(x).to(DEV), torch.from_numpy(y).to(DEV),
+                              dyadic_order=i % 3))
+    srv = S.Server(DEV)
+    srv.run(reqs, realtime=False)
+    torch.cuda.synchronize()
+    for r in reqs:
+        x, y = r.x.cpu().numpy(), r.y.cpu().numpy()
+        want = (O.sig_pairwise(x, y, r.dyadic_order) if r.kind == "pairwise"
+                else O.sig_gram(x, y, r.dyadic_order))
+        np.testing.assert_allclose(r.result.cpu().numpy(), want, rtol=1e-9)
+
+
+def test_budget_serialises_and_rejects():
+    reqs = [S.Request("pairwise", quantized(4, 17, 4, i), quantized(4, 17, 4, 50 + i),
+                      dyadic_order=i % 2) for i in range(6)]
+    one = max(S.estimate_bytes("pairwise", "ref", 4, 4, 24, 24, 4, 0, 4, 4096), 1)
+    srv = S.Server(DEV, hbm_budget=one)  # room for one batch at a time
+    peak = []
+    orig = srv._launch
+
+    def launch(b):
+        orig(b)
+        peak.append(srv._inflight_bytes)
+    srv._launch = launch
+    srv.run(reqs, realtime=False)
+    torch.cuda.synchronize()
+    assert all(r.error is None for r in reqs)
+    assert max(peak) <= one
+    big = S.Request("gram", quantized(64, 300, 64, 9), None)
+    srv2 = S.Server(DEV, hbm_budget=1 << 20)
+    srv2.run([big], realtime=False)
+    assert isinstance(big.error, MemoryError)
+
+
+@pytest.mark.parametrize("prec", ["fp16x2", "fp32"])
+def test_graph_replays_equal_eager(prec):
+    k = sig.kernels.LinearKernel()
+    srv = S.Server(DEV, use_graphs=True, streams=1)
+    for rep in range(4):
+        reqs = [S.Request("pairwise", quantized(3, 17, 4, 10 * rep + i),
+                          quantized(3, 17, 4, 10 * rep + i + 5), dyadic_order=1, precision=prec)
+                for i in range(3)]
+        srv.run(reqs, realtime=False)
+        torch.cuda.synchronize()
+        # eager work between replays
+        eager = [sig.SigPDE(k, 1, prec).pairwise(r.x, r.y) for r in reqs]
+        torch.cuda.synchronize()
+        for r, e in zip(reqs, eager):
+            assert r.error is None, r.error
+            assert torch.equal(r.result, e), (prec, rep)
+    assert len(srv._graphs) == 1  # one shape class, one padded size: captured once
+
+
+def test_summary_has_latency_percentiles():
+    reqs = S.synthetic_requests(12, seed=5, device=DEV, max_batch_size=8, lengths=(16, 40),
+                                orders=(0, 1))
+    srv = S.Server(DEV)
+    import time
+    t0 = time.perf_counter()
+    srv.run(reqs, realtime=False)
+    torch.cuda.synchronize()
+    s = S.summarize(reqs, time.perf_counter() - t0)
+    assert s["failed"] == 0 and s["requests"] == 12
+    assert 0 < s["latency_ms"]["p50"] <= s["latency_ms"]["p99"]
