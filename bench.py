@@ -61,6 +61,11 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline / accuracy leg")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--sample", type=int, default=32, help="side of the sampled accuracy block")
+    p.add_argument("--no-compare", action="store_true",
+                   help="time only --mode (skip the plain-FP32 and reference-scheme legs)")
+    p.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                   help="only run a plain device-to-device copy of BYTES bytes 5 times (the "
+                        "known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     return p.parse_args()
 
 
@@ -222,6 +227,14 @@ def main():
                                    "(multi-GPU sharding is design-only)"}))
         sys.exit(2)
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe // 4, device=dev, dtype=torch.float32)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 4}), flush=True)
+        return
     X = brownian_paths(args.paths, args.length, args.dim, 1.0 / args.length, 5, torch.float32, dev)
     Y = brownian_paths(args.paths, args.length, args.dim, 1.0 / args.length, 6, torch.float32, dev)
     torch.cuda.synchronize()
@@ -229,8 +242,11 @@ def main():
     K, ms, step_ms = timed(args, X, Y, args.mode, args.steps, args.warmup)
     pairs = X.shape[0] * Y.shape[0]
     # comparison paths on the same workload
-    K32, ms32, _ = timed(args, X, Y, "fp32", args.steps, 1) if args.mode != "fp32" else (K, ms, step_ms)
-    Kref, ms_ref, _ = timed(args, X, Y, "ref", args.ref_steps, 0) if args.mode != "ref" else (K, ms, step_ms)
+    compare = not args.no_compare
+    K32, ms32, _ = (timed(args, X, Y, "fp32", args.steps, 1) if args.mode != "fp32" and compare
+                    else (K, ms, step_ms))
+    Kref, ms_ref, _ = (timed(args, X, Y, "ref", args.ref_steps, 0) if args.mode != "ref" and compare
+                       else (K, ms, step_ms))
 
     roof = (fused_roofline(args, X, Y, ms) if args.mode != "ref"
             else dominant_kernel_roofline(args, X, Y))

@@ -21,7 +21,6 @@
 
 #include "capi_util.h"
 #include "mfma_producer.h"
-#include "ring_solver.h"
 
 namespace sigpde {
 
@@ -139,7 +138,7 @@ __global__ __launch_bounds__(64) void k_linear_gram_fused(GramFusedArgs<TO> a,
     P.ysl = 1.0;
     P.pmY = 1.0;
   }
-  ring_solve<A, G>(P, geo, 1.0, smem_raw, [&](int sp, double r) {
+  ring16_solve<A>(P, geo, 1.0, smem_raw, [&](int sp, double r) {
     const int64_t y = y0 + sp;
     if (xme >= a.Bx || y >= a.By) return;
     if (a.symmetric) {
@@ -250,7 +249,7 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
   a.ldo = ldo;
   const double *sxd = sx ? w.sxd : nullptr;
   const double *syd = sy ? w.syd : nullptr;
-  const size_t shmem = wave_lds_bytes<Arith::Mixed, 16>((int)((Ly - 1) << order));  // = Ref32's
+  const size_t shmem = r16_lds_bytes<Arith::Mixed>((int)((Ly - 1) << order));  // = Ref32's
   if (pairwise) {
     return launch_linear_pairwise_fused<T, KB, PREC>(a, sxd, syd, shmem, st);
   } else {

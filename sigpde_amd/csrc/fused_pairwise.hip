@@ -50,7 +50,7 @@ __global__ __launch_bounds__(64) void k_linear_pairwise_fused(GramFusedArgs<TO> 
     }
     P.pmY = 1.0;
   }
-  ring_solve<A, G>(P, geo, 1.0, smem_raw, [&](int, double r) {
+  ring16_solve<A>(P, geo, 1.0, smem_raw, [&](int, double r) {
     if (bme < a.Bx) a.out[bme] = (TO)r;
   });
 }

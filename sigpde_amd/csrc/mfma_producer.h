@@ -7,7 +7,7 @@
 
 #include <type_traits>
 
-#include "ring_solver.h"
+#include "ring16.h"
 
 namespace sigpde {
 
@@ -32,8 +32,9 @@ constexpr int kPrecFp32 = 2;    // fp32 increments, v_mfma_f32_16x16x4f32
 //   fp32:   lane l supplies elements (l/16)*dpad/4 + j, j < dpad/4, one per MFMA (any split of
 //           K works as long as A and B use the same one); the tiles are scaled by
 //           kscale/4^order (x_scale y_scale) in fp32 and the consumer runs Ref32.
-// Pipeline of period p: at its start, multiply chunk p+1 and fetch chunk p+2's B operands; before
-// its last consumer group, write chunk p+1's tiles (the accumulators never live across periods).
+// Pipeline of period p: at its start, multiply chunk p+1 and fetch chunk p+2's B operands; between
+// its consumer groups 1 and 2, write chunk p+1's tiles to the skewed ring of ring16.h (the
+// accumulators never live across periods).
 // After the last chunk of a band is multiplied, the next band's A operands are loaded. Operand
 // rows are addressed as a wave-uniform 64-bit row base plus a 32-bit per-lane offset.
 template <int KB, bool kSharedY, int PREC>
@@ -66,7 +67,7 @@ struct MfmaProducer {
   float pm[NP];                          // fp32: scale of the pair being multiplied
   int64_t yrow[NB];                      // first B row (in elements) of the pair being fetched
   Cursor<G> cB, cC;                      // next chunk to fetch B for / to multiply
-  int slotS;                             // ring slot of the next chunk to store
+  Ring16Writer<float> W;                 // chunk writes into the skewed ring
 
   __device__ __forceinline__ double lane_scale() const { return kSharedY ? xsl * pmY : xsl * ysl; }
 
@@ -166,40 +167,25 @@ struct MfmaProducer {
     cC.advance(geo);
     if (more && cC.jv0 == 0) fetchA(geo, cC.band);
   }
+  template <int PAR>
   __device__ __forceinline__ void store(float *ring) {
-    const int lane = threadIdx.x & 63;
-    const int slot0 = slotS;
-    const int c = lane & 15;
+    float v[NP][4];
 #pragma unroll
-    for (int g = 0; g < NP; ++g) {
-      float *dst = ring + (slot0 + c) * kRingStride + g * G + 4 * (lane >> 4);
-      float2 v0 = make_float2(acc[g][0], acc[g][1]);
-      float2 v1 = make_float2(acc[g][2], acc[g][3]);
-      if constexpr (!kF16) {
-        v0 = make_float2(v0.x * mult[g], v0.y * mult[g]);
-        v1 = make_float2(v1.x * mult[g], v1.y * mult[g]);
-      }
-      *reinterpret_cast<float2 *>(dst) = v0;
-      *reinterpret_cast<float2 *>(dst + 2) = v1;
-      if (slot0 == 0 && c < kRingMirror) {
-        float *mir = ring + (Layout<G>::kRingW + c) * kRingStride + g * G + 4 * (lane >> 4);
-        *reinterpret_cast<float2 *>(mir) = v0;
-        *reinterpret_cast<float2 *>(mir + 2) = v1;
-      }
-    }
-    slotS += kChunk;
-    if (slotS == Layout<G>::kRingW) slotS = 0;
+    for (int g = 0; g < NP; ++g)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[g][e] = kF16 ? acc[g][e] : acc[g][e] * mult[g];
+    W.template store<PAR>(ring, v);
   }
 
   __device__ __forceinline__ void prologue(float *ring, const PairGeom<G> &geo) {
     const int nchunks = geo.columns() / kChunk;
+    W.init();
     cB.start(geo, 0);
     cC.start(geo, 0);
-    slotS = 0;
     fetchA(geo, 0);
     fetchB(geo);
     compute(geo, nchunks > 1);
-    store(ring);
+    store<0>(ring);
     if (nchunks > 1) fetchB(geo);
   }
   // start of period p: multiply chunk p+1, fetch chunk p+2's B operands
@@ -208,10 +194,11 @@ struct MfmaProducer {
     if (per + 1 < nchunks) compute(geo, per + 2 < nchunks);
     if (per + 2 < nchunks) fetchB(geo);
   }
-  // before the last consumer group of period p: write chunk p+1
+  // between consumer groups 1 and 2 of period p: write chunk p+1 (ring parity PAR)
+  template <int PAR>
   __device__ __forceinline__ void period(float *ring, const PairGeom<G> &geo, int per) {
     const int nchunks = geo.columns() / kChunk;
-    if (per + 1 < nchunks) store(ring);
+    if (per + 1 < nchunks) store<PAR>(ring);
   }
 };
 

@@ -1,0 +1,342 @@
+// Goursat PDE wavefront for lane groups of 16 (four pairs per wave) with a skewed LDS ring.
+//
+// Same scheme, cell arithmetic and virtual-column stream as ring_solver.h (stojl/SigPDE
+// sigpde/cuda_device_functions.py:3-5, sigpde/cuda_pairwise_kernels.py:8-39): lane rho of a group
+// owns refined row 16*b + rho of band b and handles virtual column v at step t = v + rho; the
+// value above arrives from lane rho-1 by DPP (row_shr:1); bands of a pair and pairs of a stream
+// follow each other as one stream of virtual columns, Np per band.
+//
+// What differs is the bookkeeping, laid out so that the hot loop has no address arithmetic:
+//   * Skewed ring. Increment (row r, virtual column v) of the wave's 64 ring rows (4 pairs x 16)
+//     lives in slot (v + (r & 15)) mod 32, element r of that slot (slots of kR16Stride elements).
+//     At step t every lane reads slot t mod 32, element `lane`: one address per lane for the whole
+//     solve, the slot an immediate offset. Chunk k (16 columns) is written in period k-1, after
+//     the reads of that period were issued (its slots held chunk k-2, last read at step 16k-2).
+//     Ring and boundary inputs are loaded two groups (8 steps) before their use.
+//   * Period loop unrolled by two (even / odd parity), so ring slots are compile-time constants.
+//   * Band boundary line indexed by step: the group's last lane writes its value of step T at
+//     index T mod Np; the group's first lane reads, at step T, index (T + 15) mod Np, written at
+//     step T + 15 - Np (the previous band's value in the same column). Indices Np..Np+14 mirror
+//     0..14 (written in the periods that start a band), so a period's reads never wrap.
+//   * Three period flavours chosen once per period: Plain, Ones (the boundary values written are
+//     ones: the group's last lane is in a pair's last band, whose next band is the next pair's
+//     first), Special (a reset column, where k = 1, and/or the result cell).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ring_solver.h"
+
+namespace sigpde {
+
+constexpr int kR16Slots = 32;   // ring slots (skew 16 + one chunk of 16)
+constexpr int kR16Stride = 65;  // elements per slot: 64 rows + 1 pad (conflict-free chunk writes)
+
+// LDS for one wave: four boundary lines of Np + 16 doubles, then the ring.
+template <Arith A>
+__host__ __device__ inline size_t r16_lds_bytes(int N) {
+  using RT = typename ArithTraits<A>::RT;
+  const int np = Layout<16>::band_len(N);
+  return (size_t)4 * (np + 16) * sizeof(double) + (size_t)kR16Slots * kR16Stride * sizeof(RT);
+}
+
+// Per-lane byte offsets of a producer's chunk writes. The producing lane l = (c, h), c = l & 15,
+// h = l >> 4, holds element e (0..3) of pair g: ring row 16g + 4h + e, chunk column c. Chunk k
+// goes to slot (16 (k & 1) + c + 4h + e) & 31.
+template <typename RT>
+struct Ring16Writer {
+  uint32_t off[2][4];
+  __device__ __forceinline__ void init() {
+    const int lane = threadIdx.x & 63;
+    const int c = lane & 15, h = lane >> 4;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int slot = (16 * p + c + 4 * h + e) & 31;
+        off[p][e] = (uint32_t)((slot * kR16Stride + 4 * h + e) * sizeof(RT));
+      }
+  }
+  // v[g][e]: the lane's four values of pair g
+  template <int PAR>
+  __device__ __forceinline__ void store(RT *ring, const RT (&v)[4][4]) const {
+    char *base = reinterpret_cast<char *>(ring);
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        *reinterpret_cast<RT *>(base + off[PAR][e] + g * 16 * sizeof(RT)) = v[g][e];
+  }
+};
+
+// Producer of increments read from global memory (any static kernel; torch computed them).
+// Group g serves the pair whose increments (m x n, row stride ld) start at inc[g] (null: an idle
+// slot, served zeros). Lane (c, h) loads element e of pair g = ring row 16g + 4h + e (refined row
+// 16 band + 4h + e) at chunk column c: four 64-byte row segments per load. Chunk k is loaded two
+// periods before its use and scaled when stored.
+template <typename RT>
+struct Global16Producer {
+  const RT *inc[4];
+  RT scale[4];
+  int64_t ld;
+  bool scaled;
+  RT buf[4][4];
+  Cursor<16> cl;  // next chunk to load
+  Ring16Writer<RT> W;
+
+  __device__ __forceinline__ void load(const PairGeom<16> &g) {
+    const int lane = threadIdx.x & 63;
+    const int c = lane & 15, h = lane >> 4;
+    int jo = (cl.jv0 + c - 1) >> g.order;
+    jo = jo < 0 ? 0 : (jo > g.n - 1 ? g.n - 1 : jo);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      int ro = (cl.band * 16 + 4 * h + e) >> g.order;
+      ro = ro > g.m - 1 ? g.m - 1 : ro;
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const RT *base = inc[p];
+        buf[p][e] = base ? __builtin_nontemporal_load(base + (int64_t)ro * ld + jo) : (RT)0;
+      }
+    }
+    cl.advance(g);
+  }
+  template <int PAR>
+  __device__ __forceinline__ void store(RT *ring) {
+    RT v[4][4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[p][e] = scaled ? buf[p][e] * scale[p] : buf[p][e];
+    W.template store<PAR>(ring, v);
+  }
+  __device__ __forceinline__ void prologue(RT *ring, const PairGeom<16> &g) {
+    W.init();
+    cl.start(g, 0);
+    load(g);
+    store<0>(ring);
+    if (g.columns() > kChunk) load(g);
+  }
+  __device__ __forceinline__ void early(RT *, const PairGeom<16> &, int) {}
+  // between consumer groups 1 and 2 of period per: write chunk per+1 (parity PAR), load per+2
+  template <int PAR>
+  __device__ __forceinline__ void period(RT *ring, const PairGeom<16> &g, int per) {
+    const int nchunks = g.columns() / kChunk;
+    if (per + 1 < nchunks) store<PAR>(ring);
+    if (per + 2 < nchunks) load(g);
+  }
+  __device__ __forceinline__ double lane_scale() const { return 1.0; }
+};
+
+// Inputs of one consumer group (4 steps): ring values and boundary values.
+template <Arith A>
+struct R16In {
+  typename ArithTraits<A>::RT c[4];
+  double bv[4];
+};
+
+template <Arith A>
+struct R16State {
+  double k, kPrev;
+  double h, q;    // Mixed: coefficients of the lane's current pair
+  double hN, qN;  // Mixed: coefficients taken at the lane's next reset
+};
+
+template <bool R, bool O, bool C>
+struct R16Flavour {
+  static constexpr bool kReset = R;    // a reset column may fall in the period (runtime ones)
+  static constexpr bool kOnes = O;     // the boundary values written are ones
+  static constexpr bool kCapture = C;  // the result cell may fall in the period
+};
+using R16Plain = R16Flavour<false, false, false>;
+using R16Ones = R16Flavour<false, true, false>;
+using R16Special = R16Flavour<true, false, true>;
+
+template <Arith A>
+__device__ __forceinline__ void r16_load(const char *rb, int slot0, const double *bp,
+                                         R16In<A> &in) {
+  using RT = typename ArithTraits<A>::RT;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    in.c[q] = *reinterpret_cast<const RT *>(rb + (slot0 + q) * kR16Stride * sizeof(RT));
+#pragma unroll
+  for (int q = 0; q < 4; ++q) in.bv[q] = bp[q];
+}
+
+// Per-period pointers and flags.
+struct R16Period {
+  const double *bvCur;   // boundary line + rst0 + 15 (reads of this period)
+  const double *bvNext;  // boundary line + rst0' + 15 (first group of the next period)
+  double *w;             // boundary line + rst0 (writes of the group's last lane)
+  double *wm;            // boundary line + Np + rst0 (mirror writes; reset periods only)
+  bool ones;             // Special: the values written are ones
+  int sCap;              // Special: step of the result cell in this period, or -1
+  int rsi;               // Special: step at which this lane resets (its rho), or -1
+};
+
+// Group I (4 steps) of a period with ring parity PAR (slots 16 PAR + s). Inputs are loaded two
+// groups ahead: group I loads group I+2, which for I >= 2 is group I-2 of the next period (ring
+// parity 1 - PAR; those loads follow the producer's write of chunk per+1, made between groups 1
+// and 2, after the reads of groups 2 and 3 of this period were issued).
+template <Arith A, int PAR, class F, int I>
+__device__ __forceinline__ void r16_group(const char *rb, const R16Period &pp, double s,
+                                          R16State<A> &st, R16In<A> (&q)[2], double &result) {
+  const int lane = threadIdx.x & 63;
+  const int rho = lane & 15;
+  R16In<A> cur = q[0];
+  q[0] = q[1];
+  if constexpr (I < 2)
+    r16_load<A>(rb, 16 * PAR + 4 * (I + 2), pp.bvCur + 4 * (I + 2), q[1]);
+  else
+    r16_load<A>(rb, 16 * (1 - PAR) + 4 * (I - 2), pp.bvNext + 4 * (I - 2), q[1]);
+  double vals[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int sidx = 4 * I + k;
+    double cA, cB;
+    coefs<A>(cur.c[k], s, st.h, st.q, cA, cB);
+    const double up = shift_down<16>(st.k, cur.bv[k]);
+    double kn = cell<A>(up, st.k, st.kPrev, cA, cB);
+    st.kPrev = up;
+    if constexpr (F::kReset) {
+      const bool z = pp.rsi == sidx;
+      kn = z ? 1.0 : kn;
+      if constexpr (A == Arith::Mixed) {
+        st.h = z ? st.hN : st.h;
+        st.q = z ? st.qN : st.q;
+      }
+    }
+    st.k = kn;
+    if constexpr (F::kCapture) {
+      if (sidx == pp.sCap) result = kn;
+    }
+    vals[k] = kn;
+  }
+  if (rho == 15) {
+    double *w = pp.w + 4 * I;
+    if constexpr (F::kOnes) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) w[k] = 1.0;
+    } else if constexpr (F::kReset) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double v = pp.ones ? 1.0 : vals[k];
+        w[k] = v;
+        if (pp.wm) pp.wm[4 * I + k] = v;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) w[k] = vals[k];
+    }
+  }
+}
+
+template <Arith A, int PAR, class F, bool kSecondHalf>
+__device__ __forceinline__ void r16_half(const char *rb, const R16Period &pp, double s,
+                                         R16State<A> &st, R16In<A> (&q)[2], double &result) {
+  r16_group<A, PAR, F, kSecondHalf ? 2 : 0>(rb, pp, s, st, q, result);
+  r16_group<A, PAR, F, kSecondHalf ? 3 : 1>(rb, pp, s, st, q, result);
+}
+
+// Solves the wave's stream (g.stream pairs per lane group, one after another). For every pair of
+// the stream, out(sp, value) is called by the lanes rho == (M-1) mod 16 with that group's k[M][N].
+//   P: producer (Global16Producer or the MFMA producer): prologue / early / period<PAR> /
+//      lane_scale (Arith::Mixed: the scale of the pair whose chunk was multiplied last, read at
+//      the start of the period that holds that pair's first reset)
+//   s: per-lane double scale of the Scaled* modes (ignored otherwise)
+//   lds: r16_lds_bytes<A>(N) bytes of this wave's LDS
+template <Arith A, class Producer, class Out>
+__device__ void ring16_solve(Producer &P, const PairGeom<16> &g, double s, void *lds, Out &&out) {
+  using RT = typename ArithTraits<A>::RT;
+  const int lane = threadIdx.x & 63;
+  const int rho = lane & 15;
+  const int Np = g.Np;
+  const int bs = Np + 16;
+  double *bnd = reinterpret_cast<double *>(lds);
+  RT *ring = reinterpret_cast<RT *>(bnd + 4 * bs);
+  double *line = bnd + (lane >> 4) * bs;
+  const char *rb = reinterpret_cast<const char *>(ring + lane);
+
+  for (int i = lane; i < kR16Slots * kR16Stride; i += 64) ring[i] = (RT)0;
+  for (int i = lane; i < 4 * bs; i += 64) bnd[i] = 1.0;
+
+  const int total = g.columns();
+  const int nperiods = total / kChunk + 1;  // the last lane finishes at step total + 14
+  const int pc = g.pair_columns();
+  const int bM = (g.M - 1) >> 4;
+  const int rM = (g.M - 1) & 15;
+  int tM = bM * Np + g.N + rM;  // result step of stream pair sp
+  int sp = 0;
+
+  P.prologue(ring, g);
+
+  R16State<A> st;
+  st.k = st.kPrev = 1.0;
+  st.h = st.q = st.hN = st.qN = 0.0;
+  R16In<A> q[2];  // inputs of the next two groups
+  r16_load<A>(rb, 0, line + 15, q[0]);
+  r16_load<A>(rb, 4, line + 19, q[1]);
+
+  int rst0 = 0;   // t0 mod Np
+  int band = 0;   // band of column t0 within its pair
+  for (int per = 0; per < nperiods; ++per) {
+    const bool reset = rst0 == 0;
+    const int rn = rst0 + kChunk == Np ? 0 : rst0 + kChunk;
+    R16Period pp;
+    pp.bvCur = line + rst0 + 15;
+    pp.bvNext = line + rn + 15;
+    pp.w = line + rst0;
+    pp.wm = reset ? line + Np : nullptr;
+    // the group's last lane is in the columns t0-15 .. t0; the band of t0-1 decides
+    pp.ones = reset ? band == 0 : band == g.nb - 1;
+    const bool cap = sp < g.stream && (tM >> 4) == per;
+    pp.sCap = cap ? tM & 15 : -1;
+    pp.rsi = reset ? rho : -1;
+    if constexpr (A == Arith::Mixed) {
+      if (reset && band == 0) {
+        const double m = P.lane_scale();
+        st.hN = 0.5 * m;
+        st.qN = m * m * (1.0 / 12);
+      }
+    }
+    double result = 0.0;
+    const bool odd = per & 1;
+    const int fl = (reset || cap) ? 2 : (pp.ones ? 1 : 0);
+    P.early(ring, g, per);
+    // groups 0, 1
+    if (fl == 2) {
+      if (odd) r16_half<A, 1, R16Special, false>(rb, pp, s, st, q, result);
+      else r16_half<A, 0, R16Special, false>(rb, pp, s, st, q, result);
+    } else if (fl == 1) {
+      if (odd) r16_half<A, 1, R16Ones, false>(rb, pp, s, st, q, result);
+      else r16_half<A, 0, R16Ones, false>(rb, pp, s, st, q, result);
+    } else {
+      if (odd) r16_half<A, 1, R16Plain, false>(rb, pp, s, st, q, result);
+      else r16_half<A, 0, R16Plain, false>(rb, pp, s, st, q, result);
+    }
+    // chunk per+1 (parity of per+1) goes to the ring after the reads of groups 2, 3 were issued
+    if (odd) P.template period<0>(ring, g, per);
+    else P.template period<1>(ring, g, per);
+    // groups 2, 3
+    if (fl == 2) {
+      if (odd) r16_half<A, 1, R16Special, true>(rb, pp, s, st, q, result);
+      else r16_half<A, 0, R16Special, true>(rb, pp, s, st, q, result);
+    } else if (fl == 1) {
+      if (odd) r16_half<A, 1, R16Ones, true>(rb, pp, s, st, q, result);
+      else r16_half<A, 0, R16Ones, true>(rb, pp, s, st, q, result);
+    } else {
+      if (odd) r16_half<A, 1, R16Plain, true>(rb, pp, s, st, q, result);
+      else r16_half<A, 0, R16Plain, true>(rb, pp, s, st, q, result);
+    }
+    if (cap) {
+      if (rho == rM) out(sp, result);
+      ++sp;
+      tM += pc;
+    }
+    rst0 = rn;
+    if (rn == 0 && ++band == g.nb) band = 0;
+  }
+}
+
+}  // namespace sigpde

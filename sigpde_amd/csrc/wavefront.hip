@@ -8,6 +8,9 @@
 #include <string.h>
 
 #include "capi_util.h"
+#include <type_traits>
+
+#include "ring16.h"
 #include "ring_solver.h"
 
 namespace sigpde {
@@ -44,6 +47,19 @@ struct RefArith<double> {
   static constexpr Arith kPlain = Arith::Ref64, kScaled = Arith::Scaled64;
 };
 
+// Groups of 16 lanes run the skewed-ring solver (ring16.h), groups of 64 the generic one.
+template <typename T, int G>
+using ProducerFor = typename std::conditional<G == 16, Global16Producer<T>, GlobalProducer<T, G>>::type;
+
+template <Arith A, int G, class Producer, class Out>
+__device__ __forceinline__ void solve_wave(Producer &P, const PairGeom<G> &g, double s, void *lds,
+                                           Out &&out) {
+  if constexpr (G == 16)
+    ring16_solve<A>(P, g, s, lds, out);
+  else
+    ring_solve<A, G>(P, g, s, lds, out);
+}
+
 // One wave per block; a wave solves Layout<G>::kPairs pairs (one per lane group of G lanes).
 // Increments stream from HBM through the wave's LDS ring.
 template <typename T, int G>
@@ -52,7 +68,7 @@ __global__ __launch_bounds__(kWave) void k_pde_pairwise(PairwiseArgs<T> a) {
   using L = Layout<G>;
   const int lane = threadIdx.x & (kWave - 1);
   PairGeom<G> g(a.m, a.n, a.order);
-  GlobalProducer<T, G> P;
+  ProducerFor<T, G> P;
   P.ld = a.n;
   P.scaled = a.sx || a.sy;
 #pragma unroll
@@ -63,7 +79,7 @@ __global__ __launch_bounds__(kWave) void k_pde_pairwise(PairwiseArgs<T> a) {
     P.scale[q] = (P.scaled && ok) ? (a.sy ? a.sy[b] : (T)1) * (a.sx ? a.sx[b] : (T)1) : (T)1;
   }
   const int64_t b = (int64_t)blockIdx.x * L::kPairs + lane / G;
-  ring_solve<RefArith<T>::kPlain, G>(P, g, 1.0, smem_raw, [&](int, double r) {
+  solve_wave<RefArith<T>::kPlain, G>(P, g, 1.0, smem_raw, [&](int, double r) {
     if (b < a.B) a.out[b] = (T)r;
   });
 }
@@ -89,7 +105,7 @@ __global__ __launch_bounds__(kWave) void k_pde_gram(GramArgs<T> a, int64_t npair
   using L = Layout<G>;
   const int lane = threadIdx.x & (kWave - 1);
   PairGeom<G> g(a.m, a.n, a.order);
-  GlobalProducer<T, G> P;
+  ProducerFor<T, G> P;
   P.ld = a.n;
   P.scaled = a.sx || a.sy;
   int64_t gi[L::kPairs], gj[L::kPairs];
@@ -114,7 +130,7 @@ __global__ __launch_bounds__(kWave) void k_pde_gram(GramArgs<T> a, int64_t npair
       xi = gi[w];
       yj = gj[w];
     }
-  ring_solve<RefArith<T>::kPlain, G>(P, g, 1.0, smem_raw, [&](int, double r) {
+  solve_wave<RefArith<T>::kPlain, G>(P, g, 1.0, smem_raw, [&](int, double r) {
     if (xi >= 0) {
       a.out[xi * a.ldo + yj] = (T)r;
       if (a.symmetric && xi != yj) a.out[yj * a.ldo + xi] = (T)r;
@@ -173,7 +189,7 @@ __global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a) {
   const bool live = b < a.B;
   PairGeom<G> g(a.m, a.m, a.order);
   const int rM = (g.M - 1) & (G - 1);
-  GlobalProducer<T, G> P;
+  ProducerFor<T, G> P;
   P.ld = a.m;
   P.scaled = false;
 #pragma unroll
@@ -187,7 +203,7 @@ __global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a) {
   if (a.normalizer == 0) {
     // _norm_factors (sigpde/torch/kernel.py:181-184): K(x,x), NaN -> inf, target
     double r = 0.0;
-    ring_solve<RefArith<T>::kPlain, G>(P, g, 1.0, smem_raw, [&](int, double v) { r = v; });
+    solve_wave<RefArith<T>::kPlain, G>(P, g, 1.0, smem_raw, [&](int, double v) { r = v; });
     K1 = (T)group_bcast<G>(r, rM);
     if (isnan(K1)) K1 = (T)INFINITY;
     Tt = log_normalizer<T>(K1);
@@ -206,7 +222,7 @@ __global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a) {
   while (__any(active)) {
     double xt = av + t * (bv - av);
     double r = 0.0;
-    ring_solve<RefArith<T>::kScaled, G>(P, g, xt * xt, smem_raw, [&](int, double v) { r = v; });
+    solve_wave<RefArith<T>::kScaled, G>(P, g, xt * xt, smem_raw, [&](int, double v) { r = v; });
     double sol = group_bcast<G>(r, rM);
     if (!active) continue;
     ++it;
@@ -255,6 +271,7 @@ constexpr int64_t kManyPairs = 2048;
 
 template <typename T, int G>
 static size_t lds_for(int N) {
+  if (G == 16) return r16_lds_bytes<RefArith<T>::kPlain>(N);
   return wave_lds_bytes<RefArith<T>::kPlain, G>(N);
 }
 
