@@ -18,6 +18,10 @@ Besides the JSON contract fields the line carries:
   roofline       dominant kernel: achieved vs peak, measured with HIP events on its stream
   cpu_baseline   the oracle (C restatement, float32 reference semantics) timed on this host's
                  cores on a sampled block, in pairs/sec
+  serving        the continuous-batching request loop (sigpde_amd.serving) on a bounded sample of
+                 the request mix (70% SigPDE / 30% RobustSigPDE, pairwise or Gram, batch 1-256,
+                 length 16-512, dim 4/16/64, dyadic order 0-3, float32 Brownian paths, 64 GB HBM
+                 budget), Poisson arrivals: p50 / p99 request latency beside pairs/sec
 """
 from __future__ import annotations
 
@@ -63,6 +67,10 @@ def parse():
     p.add_argument("--sample", type=int, default=32, help="side of the sampled accuracy block")
     p.add_argument("--no-compare", action="store_true",
                    help="time only --mode (skip the plain-FP32 and reference-scheme legs)")
+    p.add_argument("--serving", type=int, default=80, metavar="N",
+                   help="requests of the serving leg (0: skip it)")
+    p.add_argument("--serving-rate", type=float, default=5.0,
+                   help="Poisson arrival rate of the serving leg, requests per second")
     p.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                    help="only run a plain device-to-device copy of BYTES bytes 5 times (the "
                         "known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -220,6 +228,32 @@ def cpu_leg(args, X, Y, K, others):
     }
 
 
+def serving_leg(args, dev):
+    """Serves a bounded sample of the request mix with Poisson arrivals; every SigPDE and robust
+    final solve runs in precision fp16x2 (the robust root-find in the reference's arithmetic on the
+    float32 increments)."""
+    from sigpde_amd import serving as S
+
+    warm = S.synthetic_requests(6, seed=99, device=dev, precision="fp16x2", max_batch_size=16,
+                                lengths=(16, 64))
+    S.Server(dev, streams=4).run(warm, realtime=False)
+    torch.cuda.synchronize()
+    reqs = S.synthetic_requests(args.serving, seed=7, rate=args.serving_rate, device=dev,
+                                precision="fp16x2")
+    torch.cuda.synchronize()
+    srv = S.Server(dev, hbm_budget=64 << 30, streams=4)
+    t0 = time.perf_counter()
+    srv.run(reqs, realtime=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = S.summarize(reqs, wall)
+    out.update({"rate_per_s": args.serving_rate, "launches": srv.launches,
+                "mix": "70% SigPDE / 30% RobustSigPDE, pairwise or Gram 50/50, batch U{1..256}, "
+                       "length U{16..512}, dim {4,16,64}, order U{0..3} (capped at a refined side "
+                       "of 4096), float32, seed 7, precision fp16x2, 64 GB budget"})
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -287,6 +321,8 @@ def main():
         for name, rel in rel_others.items():
             line[name]["rel_err_fp64"] = rel
         line["cpu_baseline"] = cpu
+    if args.serving > 0:
+        line["serving"] = serving_leg(args, dev)
     print(json.dumps(line), flush=True)
 
 

@@ -94,14 +94,18 @@ __global__ __launch_bounds__(64) void k_linear_gram_fused(GramFusedArgs<TO> a,
   const int lane = threadIdx.x & 63;
   const int64_t nq = (a.Bx + 3) / 4;
   const int64_t nyr = (a.By + a.ys - 1) / a.ys;
-  // XCD-aware order: blocks are dealt round-robin to the 8 XCDs; give consecutive y ranges of the
-  // same x quads to one XCD so its L2 sees the same y operands.
-  const int64_t nblk = nq * nyr;
+  // XCD-aware order: blocks are dealt round-robin to the 8 XCDs; each XCD takes a contiguous run
+  // of work units, ordered in super-tiles of tx x-quads by ty y-ranges, so that the waves resident
+  // on one XCD at a time share their x and y operands in that XCD's L2.
   const int64_t bid = blockIdx.x;             // grid = 8 * per_xcd blocks
   const int64_t per_xcd = gridDim.x / 8;
   const int64_t lin = (bid % 8) * per_xcd + bid / 8;
-  if (lin >= nblk) return;
-  const int64_t xq = lin % nq, yr = lin / nq;
+  const int64_t tsz = (int64_t)a.tx * a.ty;
+  const int64_t tile = lin / tsz, w = lin - tile * tsz;
+  const int64_t ntx = (nq + a.tx - 1) / a.tx;
+  const int64_t xq = (tile % ntx) * a.tx + w % a.tx;
+  const int64_t yr = (tile / ntx) * a.ty + w / a.tx;
+  if (xq >= nq || yr >= nyr) return;
   const int64_t y0 = yr * a.ys;
   int ys = (int)((a.By - y0) < a.ys ? (a.By - y0) : a.ys);
   if (a.symmetric) {
@@ -159,6 +163,9 @@ __global__ void k_to_double(const T *src, int64_t n, double *dst) {
 }
 
 // ------------------------------------------------------------------ host side
+constexpr int kTileX = 8;   // x-quads per super-tile
+constexpr int kTileY = 32;  // y-ranges per super-tile
+
 struct Workspace {
   void *xh, *xl, *yh, *yl;  // fp16x2: hi / lo halves; fp32: floats in xh / yh (xl = yl = null)
   float *invx, *invy;       // fp16x2 only
@@ -259,8 +266,12 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
     int ys = 16;
     while (ys > 1 && nq * ((By + ys - 1) / ys) < 4096) ys >>= 1;
     a.ys = ys;
-    const int64_t nblk = nq * ((By + ys - 1) / ys);
-    // a multiple of 8 blocks so that the XCD-major block order is a bijection onto the work units
+    const int64_t nyr = (By + ys - 1) / ys;
+    // super-tiles of about one XCD's resident waves (32 CUs x 8)
+    a.tx = (int)(nq < kTileX ? nq : kTileX);
+    a.ty = (int)(nyr < kTileY ? nyr : kTileY);
+    const int64_t nblk = (nq + a.tx - 1) / a.tx * a.tx * ((nyr + a.ty - 1) / a.ty * a.ty);
+    // a multiple of 8 blocks so that the XCD-major block order covers every work unit
     const int64_t grid = (nblk + 7) / 8 * 8;
     k_linear_gram_fused<T, KB, PREC><<<dim3((unsigned)grid), dim3(64), shmem, st>>>(a, sxd, syd);
   }

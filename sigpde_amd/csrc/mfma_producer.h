@@ -209,6 +209,7 @@ struct GramFusedArgs {
   const float *invx, *invy;       // fp16x2: per-path 2^-e
   int64_t Bx, By;
   int m, n, order, dpad, ys;
+  int tx, ty;     // Gram: super-tile of x-quads by y-ranges (block order)
   double kmult;  // kscale / 4^order
   int symmetric;
   TO *out;

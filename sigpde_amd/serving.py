@@ -77,9 +77,9 @@ class Request:
             raise ValueError(f"kind must be one of {KINDS}")
         if self.static_kernel is None:
             self.static_kernel = _kernels.LinearKernel()
-        if self.kind.startswith("robust") or self.kind == "normalization":
-            if self.precision != "ref":
-                raise ValueError("robust requests run with precision='ref'")
+        if self.kind == "normalization" and self.precision != "ref":
+            raise ValueError("normalization requests run with precision='ref' (the root-find "
+                             "uses the reference's arithmetic)")
 
     @property
     def latency(self) -> float:
@@ -139,7 +139,7 @@ def estimate_bytes(kind: str, precision: str, B: int, By: int, Lx: int, Ly: int,
     if kind in ("gram", "robust_gram"):
         inputs = (B * Lx + By * Ly) * d * itemsize
         out = B * By * itemsize
-        if precision == "ref" or kind == "robust_gram":
+        if precision == "ref":
             t = math.ceil(math.sqrt(max_batch))
             tiles = min(B, t) * min(By, t) * m * n * itemsize * 2  # increments + point Gram
         else:
@@ -302,7 +302,7 @@ class Server:
         if kind == "gram":
             return [SigPDE(k, order, precision).gram(r0.x, r0.y, max_batch=self.max_batch)]
         if kind == "robust_gram":
-            rk = RobustSigPDE(k, order, r0.normalizer)
+            rk = RobustSigPDE(k, order, r0.normalizer, precision)
             return [rk.gram(r0.x, r0.y, tol=r0.tol, maxit=r0.maxit, max_batch=self.max_batch)]
         sizes = [r.x.shape[0] for r in b.reqs]
         X = torch.cat([pad_paths(r.x, Lx) for r in b.reqs]) if len(b.reqs) > 1 \
@@ -323,7 +323,7 @@ class Server:
                 else:
                     out = SigPDE(k, order, precision).pairwise(X, Y, max_batch=self.max_batch)
             else:
-                rk = RobustSigPDE(k, order, r0.normalizer)
+                rk = RobustSigPDE(k, order, r0.normalizer, precision)
                 out = rk.pairwise(X, Y, tol=r0.tol, maxit=r0.maxit, max_batch=self.max_batch)
         return list(torch.split(out, sizes))
 
@@ -444,8 +444,7 @@ def synthetic_requests(n: int, seed: int = 7, rate: float = math.inf, device="cu
             kind = "robust_pairwise" if robust else "pairwise"
         if rate < math.inf:
             t += rng.exponential(1.0 / rate)
-        out.append(Request(kind, x, y, dyadic_order=order, arrival=t,
-                           precision="ref" if robust else precision))
+        out.append(Request(kind, x, y, dyadic_order=order, arrival=t, precision=precision))
     return out
 
 

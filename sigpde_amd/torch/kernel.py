@@ -226,14 +226,23 @@ class SigPDE:
 
 
 class RobustSigPDE:
-    """Robust (normalised) signature kernel (stojl/SigPDE sigpde/torch/kernel.py:174-301)."""
+    """Robust (normalised) signature kernel (stojl/SigPDE sigpde/torch/kernel.py:174-301).
 
-    def __init__(self, static_kernel, dyadic_order: int = 0, normalizer: Optional[Callable] = None):
+    precision (an addition to the reference's signature) selects how the final scaled solves of
+    pairwise / gram run, with the same meaning as for SigPDE ("ref", "fp16x2", "fp32"; the fused
+    modes need the linear static kernel). The normalisation root-find always runs the
+    reference's arithmetic on the increments of the inputs' dtype.
+    """
+
+    def __init__(self, static_kernel, dyadic_order: int = 0, normalizer: Optional[Callable] = None,
+                 precision: str = "ref"):
         if int(dyadic_order) < 0:
             raise ValueError("dyadic_order must be >= 0")
+        _fused_scale(static_kernel, precision)
         self.static_kernel = static_kernel
         self.dyadic_order = int(dyadic_order)
         self.normalizer = log_normalizer if normalizer is None else normalizer
+        self.precision = precision
 
     def _norm_factors(self, inc, lam, knorm, normalizer, tol, maxit, iters=None):
         """sigpde/torch/kernel.py:180-195 on one chunk of self-increments (B, m, m)."""
@@ -279,15 +288,20 @@ class RobustSigPDE:
             sx = torch.zeros(nb, device=x.device, dtype=x.dtype)
             kx = torch.zeros(nb, device=x.device, dtype=x.dtype)
             self._norm_factors(x_inc, sx, kx, normalizer, tol, maxit)
+            yi = xi if symmetric else y[start:stop]
             if symmetric:
-                sy, inc = sx, x_inc
+                sy = sx
             else:
-                yi = y[start:stop]
                 y_inc = pairwise_inner_product(yi, yi, self.static_kernel, self.dyadic_order).contiguous()
                 sy = torch.zeros(nb, device=x.device, dtype=x.dtype)
                 ky = torch.zeros(nb, device=x.device, dtype=x.dtype)
                 self._norm_factors(y_inc, sy, ky, normalizer, tol, maxit)
-                inc = pairwise_inner_product(xi, yi, self.static_kernel, self.dyadic_order).contiguous()
+            if self.precision != "ref":
+                out[start:stop] = SigPDE(self.static_kernel, self.dyadic_order, self.precision) \
+                    .pairwise(xi, yi, sx, sy)
+                continue
+            inc = x_inc if symmetric else pairwise_inner_product(
+                xi, yi, self.static_kernel, self.dyadic_order).contiguous()
             _native.pde_pairwise(inc, self.dyadic_order, out[start:stop], sx, sy)
         return out
 
@@ -300,5 +314,5 @@ class RobustSigPDE:
         else:
             y_scales = self.normalization(y, normalizer, tol, maxit, max_batch, max_threads) \
                 if y_scale is None else y_scale
-        return SigPDE(self.static_kernel, self.dyadic_order).gram(x, y, x_scales, y_scales,
-                                                                  max_batch, max_threads)
+        return SigPDE(self.static_kernel, self.dyadic_order, self.precision).gram(
+            x, y, x_scales, y_scales, max_batch, max_threads)

@@ -29,7 +29,7 @@ def alone(r):
         return sig.SigPDE(k, r.dyadic_order, r.precision).pairwise(r.x, r.y)
     if r.kind == "gram":
         return sig.SigPDE(k, r.dyadic_order, r.precision).gram(r.x, r.y)
-    rk = sig.RobustSigPDE(k, r.dyadic_order, r.normalizer)
+    rk = sig.RobustSigPDE(k, r.dyadic_order, r.normalizer, r.precision)
     if r.kind == "normalization":
         return rk.normalization(r.x, tol=r.tol, maxit=r.maxit)
     if r.kind == "robust_pairwise":
@@ -53,7 +53,10 @@ def mix(seed, n=24):
             y = None
         else:
             y = quantized(B, L, d, 300 + i)
-        prec = "fp32" if (kind in ("pairwise", "gram") and i % 3 == 0) else "ref"
+        # fused precision for some SigPDE requests and every robust Gram (launched alone); the other
+        # robust kinds stay "ref" so that several requests share their shape classes
+        prec = "fp32" if ((kind in ("pairwise", "gram") and i % 3 == 0)
+                          or kind == "robust_gram") else "ref"
         reqs.append(S.Request(kind, x, y, dyadic_order=order, precision=prec,
                               tol=1e-6, maxit=60))
     return reqs
