@@ -54,7 +54,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--mode", default="fp16x2", choices=["fp16x2", "fp32", "ref"])
+    p.add_argument("--mode", default="fp16x2", choices=["fp16x2", "bf16x3", "fp32", "ref"])
     p.add_argument("--ref-steps", type=int, default=1,
                    help="timed steps of the reference-scheme FP32 path")
     p.add_argument("--paths", type=int, default=1024)
@@ -120,6 +120,8 @@ def fused_roofline(args, X, Y, ms):
     flops_inc = pairs * 2 * args.dim * (X.shape[1] - 1) * (Y.shape[1] - 1)
     name = {"fp16x2": "k_linear_gram_fused<float, 2, 1> (fp16x2 MFMA increments + double "
                       "wavefront, one launch)",
+            "bf16x3": "k_linear_gram_fused<float, 2, 3> (bf16x3 MFMA increments + double "
+                      "wavefront, one launch)",
             "fp32": "k_linear_gram_fused<float, 2, 2> (fp32 MFMA increments + float32-typed "
                     "wavefront, one launch)"}[args.mode]
     return {
@@ -134,7 +136,7 @@ def fused_roofline(args, X, Y, ms):
         "cells_per_s": cells / dur_s,
         "increment_gram_tflops": flops_inc / dur_s / 1e12,
         "increment_gram_mfma_frac": flops_inc / dur_s / 1e12 / (
-            MI355X_FP16_MFMA_TFLOPS if args.mode == "fp16x2" else MI355X_FP32_MFMA_TFLOPS),
+            MI355X_FP32_MFMA_TFLOPS if args.mode == "fp32" else MI355X_FP16_MFMA_TFLOPS),
         "note": "wavefront flops = 10 per refined cell (pde_step as written); the increment Gram "
                 "(2*d flops per increment) runs on MFMA beside it",
     }
@@ -281,6 +283,8 @@ def main():
                     else (K, ms, step_ms))
     Kref, ms_ref, _ = (timed(args, X, Y, "ref", args.ref_steps, 0) if args.mode != "ref" and compare
                        else (K, ms, step_ms))
+    Kb, msb, _ = (timed(args, X, Y, "bf16x3", args.steps, 1) if args.mode != "bf16x3" and compare
+                  else (K, ms, step_ms))
 
     roof = (fused_roofline(args, X, Y, ms) if args.mode != "ref"
             else dominant_kernel_roofline(args, X, Y))
@@ -296,7 +300,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp16x2": "fp16x2+f64", "fp32": "f32", "ref": "f32"}[args.mode],
+        "dtype": {"fp16x2": "fp16x2+f64", "bf16x3": "bf16x3+f64", "fp32": "f32",
+                  "ref": "f32"}[args.mode],
         "data": "synthetic",
         "config": {"workload": f"SigPDE(LinearKernel(), {args.order}).gram(X, Y), X,Y "
                                f"{args.paths} Brownian paths, length {args.length}, dim "
@@ -308,6 +313,9 @@ def main():
         "plain_fp32": {"value": pairs / (ms32 / 1e3), "unit": "pairs/s", "ms_per_step": ms32,
                        "path": "precision='fp32': fused, fp32 MFMA increments, float32 wavefront"},
         "speedup_vs_plain_fp32": ms32 / ms,
+        "mixed_bf16x3": {"value": pairs / (msb / 1e3), "unit": "pairs/s", "ms_per_step": msb,
+                         "path": "precision='bf16x3': three-part bf16 increments, six bf16 MFMA "
+                                 "products, double wavefront"},
         "reference_scheme_fp32": {"value": pairs / (ms_ref / 1e3), "unit": "pairs/s",
                                   "ms_per_step": ms_ref,
                                   "path": "precision='ref': torch increment tiles (point Gram, "
@@ -316,7 +324,8 @@ def main():
     }
     if not args.no_cpu:
         acc, rel_others, cpu = cpu_leg(args, X, Y, K, {"plain_fp32": K32,
-                                                        "reference_scheme_fp32": Kref})
+                                                        "reference_scheme_fp32": Kref,
+                                                        "mixed_bf16x3": Kb})
         line.update(acc)
         for name, rel in rel_others.items():
             line[name]["rel_err_fp64"] = rel

@@ -99,6 +99,8 @@ int sigpde_robust_norm(int dtype, const void *inc, int64_t B, int64_t m, int64_t
  * 145-167) for the linear kernel.
  *   precision SIGPDE_PREC_FP16X2: increments split into per-path-scaled fp16 hi + lo, products
  *     accumulated in fp32 on MFMA; the wavefront runs in double; results in dtype.
+ *   precision SIGPDE_PREC_BF16X3: the same scaled increments split into three bf16 parts, the six
+ *     products of magnitude >= 2^-16 of hi*hi accumulated in fp32 on bf16 MFMA; double wavefront.
  *   precision SIGPDE_PREC_FP32: plain FP32 -- increments rounded to float, products on fp32
  *     MFMA, the wavefront in the reference's float32 typing (c, c^2 in float, the rest in double,
  *     cells stored as float); results in dtype.
@@ -111,7 +113,7 @@ int sigpde_robust_norm(int dtype, const void *inc, int64_t B, int64_t m, int64_t
  *     (the device-side increments and per-path scales); same_paths != 0 when Y is X (one split
  *     serves both). Returns 0 for invalid sizes or an unknown precision.
  */
-enum sigpde_precision { SIGPDE_PREC_FP16X2 = 1, SIGPDE_PREC_FP32 = 2 };
+enum sigpde_precision { SIGPDE_PREC_FP16X2 = 1, SIGPDE_PREC_FP32 = 2, SIGPDE_PREC_BF16X3 = 3 };
 
 size_t sigpde_linear_workspace_bytes(int precision, int64_t Bx, int64_t Lx, int64_t By, int64_t Ly,
                                      int64_t d, int same_paths);

@@ -18,7 +18,8 @@ F32, F64 = 0, 1
 NORM_LOG, NORM_GIVEN = 0, 1
 PREC_FP16X2 = 1
 PREC_FP32 = 2
-PRECISIONS = {"fp16x2": PREC_FP16X2, "fp32": PREC_FP32}
+PREC_BF16X3 = 3
+PRECISIONS = {"fp16x2": PREC_FP16X2, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
 
 _lib = None
 _lock = threading.Lock()

@@ -11,6 +11,10 @@
 // the pair's scale m = kscale 4^-order 2^-(ex+ey) (x_scale y_scale) through its coefficients
 // (Arith::Mixed). Relative error against the float64 reference: ~1e-6 on Brownian paths.
 //
+// Precision "bf16x3": the same scaled increments split into three bf16 parts (hi, mid, lo: each
+// the bf16 rounding of what the previous parts leave), the six products of magnitude >= 2^-16 of
+// hi*hi on v_mfma_f32_16x16x32_bf16 in fp32, the same double wavefront (Arith::Mixed).
+//
 // Precision "fp32" (the plain FP32 fused path): increments rounded to float, products on
 // v_mfma_f32_16x16x4f32 in fp32, scaled in fp32, and the wavefront in the reference's float32
 // typing (Arith::Ref32: c and c^2 in float, the rest in double, cells stored as float).
@@ -25,13 +29,14 @@
 namespace sigpde {
 
 // ------------------------------------------------------------------ split prepass
-// One block per path: dx = x[s+1] - x[s] in double (exact for float inputs), per-path scale,
-// fp16 hi/lo rows of dpad halves (zero padded), and inv = 2^-e.
-template <typename T>
-__global__ __launch_bounds__(256) void k_split_fp16(const T *__restrict__ X, int64_t B, int L,
-                                                    int d, int dpad, _Float16 *__restrict__ hi,
-                                                    _Float16 *__restrict__ lo,
-                                                    float *__restrict__ inv) {
+// One block per path: dx = x[s+1] - x[s] in double (exact for float inputs), per-path scale 2^e
+// (max |dx| 2^e in [2^13, 2^14): fp16 parts stay in range), NPART 16-bit parts of E per element,
+// each the rounding of what the previous parts leave (computed exactly in double), rows of dpad
+// elements (zero padded), and inv = 2^-e.
+template <typename T, typename E, int NPART>
+__global__ __launch_bounds__(256) void k_split16(const T *__restrict__ X, int64_t B, int L, int d,
+                                                 int dpad, E *__restrict__ p0, E *__restrict__ p1,
+                                                 E *__restrict__ p2, float *__restrict__ inv) {
   __shared__ double red[256];
   const int64_t b = blockIdx.x;
   const T *x = X + b * (int64_t)L * d;
@@ -51,16 +56,22 @@ __global__ __launch_bounds__(256) void k_split_fp16(const T *__restrict__ X, int
   mx = red[0];
   const int e = (mx > 0.0 && isfinite(mx)) ? 13 - ilogb(mx) : 0;
   if (threadIdx.x == 0) inv[b] = ldexpf(1.0f, -e);
-  _Float16 *h = hi + b * (int64_t)m * dpad;
-  _Float16 *l = lo + b * (int64_t)m * dpad;
+  const int64_t row0 = b * (int64_t)m * dpad;
   for (int i = threadIdx.x; i < m * dpad; i += blockDim.x) {
     const int s = i / dpad, k = i - s * dpad;
     double v = 0.0;
     if (k < d) v = ldexp((double)x[(s + 1) * d + k] - (double)x[s * d + k], e);
-    const _Float16 vh = (_Float16)v;
-    const _Float16 vl = (_Float16)(v - (double)vh);
-    h[i] = vh;
-    l[i] = vl;
+    const E a = (E)(float)v;
+    v -= (double)(float)a;
+    const E c = (E)(float)v;
+    p0[row0 + i] = a;
+    if (NPART == 2) {
+      p1[row0 + i] = c;
+    } else {
+      v -= (double)(float)c;
+      p1[row0 + i] = c;
+      p2[row0 + i] = (E)(float)v;
+    }
   }
 }
 
@@ -88,7 +99,7 @@ __global__ __launch_bounds__(64) void k_linear_gram_fused(GramFusedArgs<TO> a,
                                                           const double *sxd, const double *syd) {
   using Prod = MfmaProducer<KB, true, PREC>;
   using E = typename Prod::elt;
-  constexpr Arith A = PREC == kPrecFp16x2 ? Arith::Mixed : Arith::Ref32;
+  constexpr Arith A = PREC == kPrecFp32 ? Arith::Ref32 : Arith::Mixed;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int G = 16;
   const int lane = threadIdx.x & 63;
@@ -120,6 +131,8 @@ __global__ __launch_bounds__(64) void k_linear_gram_fused(GramFusedArgs<TO> a,
   P.xl = (const E *)a.xl;
   P.yh = (const E *)a.yh;
   P.yl = (const E *)a.yl;
+  P.xm = (const E *)a.xm;
+  P.ym = (const E *)a.ym;
   P.dpad = a.dpad;
   P.m = a.m;
   P.n = a.n;
@@ -138,7 +151,7 @@ __global__ __launch_bounds__(64) void k_linear_gram_fused(GramFusedArgs<TO> a,
   const int64_t xme = xq * 4 + lane / G;
   {
     const int64_t xc = xme < a.Bx ? xme : a.Bx - 1;
-    P.xsl = PREC == kPrecFp16x2 ? a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0) : 1.0;
+    P.xsl = PREC != kPrecFp32 ? a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0) : 1.0;
     P.ysl = 1.0;
     P.pmY = 1.0;
   }
@@ -168,7 +181,8 @@ constexpr int kTileY = 32;  // y-ranges per super-tile
 
 struct Workspace {
   void *xh, *xl, *yh, *yl;  // fp16x2: hi / lo halves; fp32: floats in xh / yh (xl = yl = null)
-  float *invx, *invy;       // fp16x2 only
+  void *xm, *ym;            // bf16x3: middle parts (hi / lo in xh / xl)
+  float *invx, *invy;       // fp16x2, bf16x3
   double *sxd, *syd;
 };
 
@@ -195,6 +209,10 @@ static size_t workspace_layout(int precision, int64_t Bx, int64_t Lx, int64_t By
     t.xl = take(nx * 2);
     t.yh = same ? t.xh : take(ny * 2);
     t.yl = same ? t.xl : take(ny * 2);
+    if (precision == SIGPDE_PREC_BF16X3) {
+      t.xm = take(nx * 2);
+      t.ym = same ? t.xm : take(ny * 2);
+    }
     t.invx = (float *)take(Bx * 4);
     t.invy = same ? t.invx : (float *)take(By * 4);
   }
@@ -205,14 +223,18 @@ static size_t workspace_layout(int precision, int64_t Bx, int64_t Lx, int64_t By
 }
 
 template <typename T, int PREC>
-static hipError_t run_split(const T *X, int64_t B, int64_t L, int64_t d, int dp, void *h, void *l,
-                            float *inv, hipStream_t st) {
+static hipError_t run_split(const T *X, int64_t B, int64_t L, int64_t d, int dp, void *h, void *mid,
+                            void *l, float *inv, hipStream_t st) {
   if (B == 0) return hipSuccess;
+  const dim3 grid((unsigned)B), block(256);
   if (PREC == kPrecFp32)
-    k_split_f32<T><<<dim3((unsigned)B), dim3(256), 0, st>>>(X, (int)L, (int)d, dp, (float *)h);
+    k_split_f32<T><<<grid, block, 0, st>>>(X, (int)L, (int)d, dp, (float *)h);
+  else if (PREC == kPrecBf16x3)
+    k_split16<T, __bf16, 3><<<grid, block, 0, st>>>(X, B, (int)L, (int)d, dp, (__bf16 *)h,
+                                                    (__bf16 *)mid, (__bf16 *)l, inv);
   else
-    k_split_fp16<T><<<dim3((unsigned)B), dim3(256), 0, st>>>(X, B, (int)L, (int)d, dp,
-                                                             (_Float16 *)h, (_Float16 *)l, inv);
+    k_split16<T, _Float16, 2><<<grid, block, 0, st>>>(X, B, (int)L, (int)d, dp, (_Float16 *)h,
+                                                      (_Float16 *)l, nullptr, inv);
   return hipGetLastError();
 }
 
@@ -229,10 +251,10 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
                                const void *sx, const void *sy, int symmetric, void *out,
                                int64_t ldo, const Workspace &w, hipStream_t st) {
   const int dp = dpad_for(d);
-  hipError_t e = run_split<T, PREC>((const T *)X, Bx, Lx, d, dp, w.xh, w.xl, w.invx, st);
+  hipError_t e = run_split<T, PREC>((const T *)X, Bx, Lx, d, dp, w.xh, w.xm, w.xl, w.invx, st);
   if (e != hipSuccess) return e;
   if (w.yh != w.xh) {
-    e = run_split<T, PREC>((const T *)Y, By, Ly, d, dp, w.yh, w.yl, w.invy, st);
+    e = run_split<T, PREC>((const T *)Y, By, Ly, d, dp, w.yh, w.ym, w.yl, w.invy, st);
     if (e != hipSuccess) return e;
   }
   if ((e = run_to_double<T>(sx, Bx, w.sxd, st)) != hipSuccess) return e;
@@ -242,6 +264,8 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
   a.xl = w.xl;
   a.yh = w.yh;
   a.yl = w.yl;
+  a.xm = w.xm;
+  a.ym = w.ym;
   a.invx = w.invx;
   a.invy = w.invy;
   a.Bx = Bx;
@@ -308,6 +332,9 @@ static hipError_t dispatch_prec(int precision, bool pairwise, const void *X, con
   if (precision == SIGPDE_PREC_FP32)
     return dispatch_kb<T, kPrecFp32>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
                                      symmetric, out, ldo, w, st);
+  if (precision == SIGPDE_PREC_BF16X3)
+    return dispatch_kb<T, kPrecBf16x3>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
+                                       symmetric, out, ldo, w, st);
   return dispatch_kb<T, kPrecFp16x2>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
                                      symmetric, out, ldo, w, st);
 }
@@ -319,7 +346,8 @@ static int check_fused(int dtype, int precision, const void *X, const void *Y, i
                        void *ws) {
   if (dtype != SIGPDE_F32 && dtype != SIGPDE_F64)
     return set_error(SIGPDE_EINVAL, "linear: unknown dtype %d", dtype);
-  if (precision != SIGPDE_PREC_FP16X2 && precision != SIGPDE_PREC_FP32)
+  if (precision != SIGPDE_PREC_FP16X2 && precision != SIGPDE_PREC_FP32 &&
+      precision != SIGPDE_PREC_BF16X3)
     return set_error(SIGPDE_EINVAL, "linear: unsupported precision mode %d", precision);
   if (Bx < 0 || By < 0 || !X || !Y || !out || !ws)
     return set_error(SIGPDE_EINVAL, "linear: bad batch or null pointer");
@@ -338,7 +366,9 @@ using namespace sigpde;
 extern "C" size_t sigpde_linear_workspace_bytes(int precision, int64_t Bx, int64_t Lx, int64_t By,
                                                int64_t Ly, int64_t d, int same_paths) {
   if (Bx < 0 || By < 0 || Lx < 2 || Ly < 2 || d < 1) return 0;
-  if (precision != SIGPDE_PREC_FP16X2 && precision != SIGPDE_PREC_FP32) return 0;
+  if (precision != SIGPDE_PREC_FP16X2 && precision != SIGPDE_PREC_FP32 &&
+      precision != SIGPDE_PREC_BF16X3)
+    return 0;
   return workspace_layout(precision, Bx, Lx, By, Ly, d, same_paths, nullptr, nullptr);
 }
 

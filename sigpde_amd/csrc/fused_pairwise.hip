@@ -13,7 +13,7 @@ __global__ __launch_bounds__(64) void k_linear_pairwise_fused(GramFusedArgs<TO> 
                                                               const double *syd) {
   using Prod = MfmaProducer<KB, false, PREC>;
   using E = typename Prod::elt;
-  constexpr Arith A = PREC == kPrecFp16x2 ? Arith::Mixed : Arith::Ref32;
+  constexpr Arith A = PREC == kPrecFp32 ? Arith::Ref32 : Arith::Mixed;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int G = 16;
   const int lane = threadIdx.x & 63;
@@ -23,6 +23,8 @@ __global__ __launch_bounds__(64) void k_linear_pairwise_fused(GramFusedArgs<TO> 
   P.xl = (const E *)a.xl;
   P.yh = (const E *)a.yh;
   P.yl = (const E *)a.yl;
+  P.xm = (const E *)a.xm;
+  P.ym = (const E *)a.ym;
   P.dpad = a.dpad;
   P.m = a.m;
   P.n = a.n;
@@ -42,7 +44,7 @@ __global__ __launch_bounds__(64) void k_linear_pairwise_fused(GramFusedArgs<TO> 
   {
     const int64_t xc = bme < a.Bx ? bme : a.Bx - 1;
     const int64_t yc = bme < a.By ? bme : a.By - 1;
-    if (PREC == kPrecFp16x2) {
+    if (PREC != kPrecFp32) {
       P.xsl = a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0);
       P.ysl = (double)a.invy[yc] * (syd ? syd[yc] : 1.0);
     } else {
@@ -67,6 +69,8 @@ hipError_t launch_linear_pairwise_fused(const GramFusedArgs<T> &a, const double 
   template hipError_t launch_linear_pairwise_fused<T, KB, kPrecFp16x2>(                          \
       const GramFusedArgs<T> &, const double *, const double *, size_t, hipStream_t);          \
   template hipError_t launch_linear_pairwise_fused<T, KB, kPrecFp32>(                            \
+      const GramFusedArgs<T> &, const double *, const double *, size_t, hipStream_t);         \
+  template hipError_t launch_linear_pairwise_fused<T, KB, kPrecBf16x3>(                          \
       const GramFusedArgs<T> &, const double *, const double *, size_t, hipStream_t);
 SIGPDE_PAIRWISE_INST(float, 1)
 SIGPDE_PAIRWISE_INST(float, 2)

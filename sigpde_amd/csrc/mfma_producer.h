@@ -12,13 +12,15 @@
 namespace sigpde {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 
-constexpr int kKBlock = 32;  // dimension padding unit (K of one v_mfma_f32_16x16x32_f16)
+constexpr int kKBlock = 32;  // dimension padding unit (K of one v_mfma_f32_16x16x32_{f16,bf16})
 
 // Operand precisions of the fused producer (values of SIGPDE_PREC_*).
 constexpr int kPrecFp16x2 = 1;  // per-path-scaled fp16 hi + lo, 3 f16 MFMAs per 32-wide K block
 constexpr int kPrecFp32 = 2;    // fp32 increments, v_mfma_f32_16x16x4f32
+constexpr int kPrecBf16x3 = 3;  // bf16 hi + mid + lo, 6 bf16 MFMAs per 32-wide K block
 
 // ------------------------------------------------------------------ MFMA producer (G = 16)
 // Lane group g (16 lanes) solves pairs (x path xa[g], y path ybase[g] + sp * ystep) of the stream.
@@ -29,6 +31,8 @@ constexpr int kPrecFp32 = 2;    // fp32 increments, v_mfma_f32_16x16x4f32
 // D[4(l/16) + e][l % 16] and writes those four ring rows of column l % 16.
 //   fp16x2: lane l supplies K block kb's elements 8(l/16)..+7 (one half8 per block, hi and lo);
 //           the tiles go to the ring unscaled (the consumer applies the pair's scale, Mixed).
+//   bf16x3: as fp16x2 with three bf16 parts per element (hi, mid, lo) and the six products
+//           whose magnitude is at least 2^-16 of hi*hi, smallest first.
 //   fp32:   lane l supplies elements (l/16)*dpad/4 + j, j < dpad/4, one per MFMA (any split of
 //           K works as long as A and B use the same one); the tiles are scaled by
 //           kscale/4^order (x_scale y_scale) in fp32 and the consumer runs Ref32.
@@ -43,10 +47,15 @@ struct MfmaProducer {
   static constexpr int NP = 4;           // pairs (lane groups) per wave
   static constexpr int NB = kSharedY ? 1 : NP;
   static constexpr bool kF16 = PREC == kPrecFp16x2;
-  static constexpr int NV = kF16 ? KB : 2 * KB;  // operand vectors per lane (half8 or float4)
-  typedef typename std::conditional<kF16, half8, float4v>::type vec;
-  typedef typename std::conditional<kF16, _Float16, float>::type elt;
+  static constexpr bool kB3 = PREC == kPrecBf16x3;
+  static constexpr bool kSplit = kF16 || kB3;     // 16-bit parts, pair scale applied by Mixed
+  static constexpr int NV = kSplit ? KB : 2 * KB;  // operand vectors per lane (8 x 16 bit or float4)
+  typedef typename std::conditional<kF16, half8,
+                                    typename std::conditional<kB3, bf16x8, float4v>::type>::type vec;
+  typedef typename std::conditional<kF16, _Float16,
+                                    typename std::conditional<kB3, __bf16, float>::type>::type elt;
   const elt *xh, *xl, *yh, *yl;          // fp32: xh / yh only
+  const elt *xm, *ym;                    // bf16x3: middle parts
   int dpad;
   int m, n;                              // original increments per x / y path
   int64_t xa[NP];                        // x path of each group (-1: idle)
@@ -56,12 +65,12 @@ struct MfmaProducer {
   const double *sx, *sy;                 // optional path scales (robust Gram), may be null
   int64_t ny;                            // valid y paths (ybase + sp*ystep < ny)
   double kmult;                          // fp32: kscale / 4^order
-  double xsl;                            // fp16x2, per lane: kscale 4^-order 2^-ex (x_scale)
-  double ysl;                            // fp16x2 pairwise, per lane: 2^-ey (y_scale)
-  double pmY;                            // fp16x2 Gram: 2^-ey (y_scale) of the pair multiplied last
+  double xsl;                            // 16-bit splits, per lane: kscale 4^-order 2^-ex (x_scale)
+  double ysl;                            // 16-bit splits, pairwise, per lane: 2^-ey (y_scale)
+  double pmY;                            // 16-bit splits, Gram: 2^-ey (y_scale) of the pair multiplied last
 
-  vec Ah[NP][NV], Al[NP][kF16 ? NV : 1];   // A operands of the band being multiplied
-  vec Bh[NB][NV], Bl[NB][kF16 ? NV : 1];   // next chunk's B operands (fetched one period ahead)
+  vec Ah[NP][NV], Al[NP][kSplit ? NV : 1], Am[NP][kB3 ? NV : 1];  // A operands of the band
+  vec Bh[NB][NV], Bl[NB][kSplit ? NV : 1], Bm[NB][kB3 ? NV : 1];  // next chunk's B operands
   float4v acc[NP];
   float mult[NP];                        // fp32: scale of the tiles in acc
   float pm[NP];                          // fp32: scale of the pair being multiplied
@@ -78,15 +87,16 @@ struct MfmaProducer {
   // per-lane element offset of this lane's K slice in an operand row
   __device__ __forceinline__ uint32_t kslice() const {
     const int h = (threadIdx.x & 63) >> 4;
-    return kF16 ? 8 * h : h * (dpad / 4);
+    return kSplit ? 8 * h : h * (dpad / 4);
   }
-  __device__ __forceinline__ void load_row(const elt *ph, const elt *pl, uint32_t lo, vec *dh,
-                                           vec *dl) {
+  __device__ __forceinline__ void load_row(const elt *ph, const elt *pm, const elt *pl, uint32_t lo,
+                                           vec *dh, vec *dm, vec *dl) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-      if constexpr (kF16) {
+      if constexpr (kSplit) {
         dh[v] = *reinterpret_cast<const vec *>(ph + lo + v * kKBlock);
         dl[v] = *reinterpret_cast<const vec *>(pl + lo + v * kKBlock);
+        if constexpr (kB3) dm[v] = *reinterpret_cast<const vec *>(pm + lo + v * kKBlock);
       } else {
         dh[v] = *reinterpret_cast<const vec *>(ph + lo + 4 * v);
       }
@@ -101,7 +111,8 @@ struct MfmaProducer {
 #pragma unroll
     for (int g = 0; g < NP; ++g) {
       const int64_t xrow = (xa[g] < 0 ? 0 : xa[g]) * m * (int64_t)dpad;
-      load_row(xh + xrow, kF16 ? xl + xrow : nullptr, lo, Ah[g], Al[g]);
+      load_row(xh + xrow, kB3 ? xm + xrow : nullptr, kSplit ? xl + xrow : nullptr, lo, Ah[g],
+               Am[g], Al[g]);
     }
   }
   // B operands of chunk cB, then advance cB.
@@ -116,7 +127,8 @@ struct MfmaProducer {
     const uint32_t lo = (uint32_t)(co * dpad) + kslice();
 #pragma unroll
     for (int q = 0; q < NB; ++q)
-      load_row(yh + yrow[q], kF16 ? yl + yrow[q] : nullptr, lo, Bh[q], Bl[q]);
+      load_row(yh + yrow[q], kB3 ? ym + yrow[q] : nullptr, kSplit ? yl + yrow[q] : nullptr, lo,
+               Bh[q], Bm[q], Bl[q]);
     cB.advance(geo);
   }
   // Tiles of chunk cC, then advance cC; if the next chunk opens a band, load that band's A
@@ -124,7 +136,7 @@ struct MfmaProducer {
   // wave computes).
   __device__ __forceinline__ void compute(const PairGeom<G> &geo, bool more) {
     if (cC.band == 0 && cC.jv0 == 0) {
-      if constexpr (kF16) {
+      if constexpr (kSplit) {
         if (kSharedY) {
           const int64_t yp = ypath(0, cC.sp);
           pmY = sy ? (double)invy[yp] * sy[yp] : (double)invy[yp];
@@ -154,6 +166,22 @@ struct MfmaProducer {
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
           a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[g][kb], Bh[q][kb], a, 0, 0, 0);
+      } else if constexpr (kB3) {
+        // hi*lo + mid*mid + lo*hi (2^-16), then hi*mid + mid*hi (2^-8), then hi*hi
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) {
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[g][kb], Bl[q][kb], a, 0, 0, 0);
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am[g][kb], Bm[q][kb], a, 0, 0, 0);
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[g][kb], Bh[q][kb], a, 0, 0, 0);
+        }
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) {
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[g][kb], Bm[q][kb], a, 0, 0, 0);
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am[g][kb], Bh[q][kb], a, 0, 0, 0);
+        }
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[g][kb], Bh[q][kb], a, 0, 0, 0);
       } else {
 #pragma unroll
         for (int v = 0; v < NV; ++v)
@@ -173,7 +201,7 @@ struct MfmaProducer {
 #pragma unroll
     for (int g = 0; g < NP; ++g)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) v[g][e] = kF16 ? acc[g][e] : acc[g][e] * mult[g];
+      for (int e = 0; e < 4; ++e) v[g][e] = kSplit ? acc[g][e] : acc[g][e] * mult[g];
     W.template store<PAR>(ring, v);
   }
 
@@ -206,7 +234,8 @@ struct MfmaProducer {
 template <typename TO>
 struct GramFusedArgs {
   const void *xh, *xl, *yh, *yl;  // fp16x2: _Float16 hi / lo rows; fp32: float rows in xh / yh
-  const float *invx, *invy;       // fp16x2: per-path 2^-e
+  const void *xm, *ym;            // bf16x3: middle parts (hi / lo in xh / xl)
+  const float *invx, *invy;       // fp16x2, bf16x3: per-path 2^-e
   int64_t Bx, By;
   int m, n, order, dpad, ys;
   int tx, ty;     // Gram: super-tile of x-quads by y-ranges (block order)

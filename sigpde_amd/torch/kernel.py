@@ -97,7 +97,7 @@ def _scale_vector(s, batch: int, like: torch.Tensor) -> torch.Tensor:
     return s.contiguous()
 
 
-PRECISIONS = ("ref", "fp16x2", "fp32")
+PRECISIONS = ("ref", "fp16x2", "bf16x3", "fp32")
 
 
 def _fused_scale(static_kernel, precision: str):
@@ -123,6 +123,8 @@ class SigPDE:
       "fp16x2" -- linear kernel only: increments <dx, dy> formed from the paths and multiplied as
                   per-path-scaled fp16 hi/lo pairs on MFMA (fp32 accumulate), double wavefront;
                   relative error ~1e-6 against the float64 reference scheme;
+      "bf16x3" -- linear kernel only: as "fp16x2" with the increments split into three bf16 parts
+                  and six bf16 MFMA products (the terms of magnitude >= 2^-16 of hi*hi);
       "fp32"   -- linear kernel only, plain FP32 fused: increments rounded to float, <dx, dy> on
                   fp32 MFMA, the wavefront in the reference's float32 typing (no increment
                   tensor in HBM, as for "fp16x2").
@@ -229,7 +231,7 @@ class RobustSigPDE:
     """Robust (normalised) signature kernel (stojl/SigPDE sigpde/torch/kernel.py:174-301).
 
     precision (an addition to the reference's signature) selects how the final scaled solves of
-    pairwise / gram run, with the same meaning as for SigPDE ("ref", "fp16x2", "fp32"; the fused
+    pairwise / gram run, with the same meaning as for SigPDE ("ref", "fp16x2", "bf16x3", "fp32"; the fused
     modes need the linear static kernel). The normalisation root-find always runs the
     reference's arithmetic on the increments of the inputs' dtype.
     """

@@ -3,6 +3,8 @@ inputs upcast:
   precision="fp16x2" (fp16 hi/lo increment Gram on MFMA, double wavefront): the headline bar,
     relative Frobenius error <= 1e-5 (measured ~1e-6), and 1e-4 on every entry whose magnitude is
     at least 1% of the largest;
+  precision="bf16x3" (three bf16 parts, six bf16 MFMA products, double wavefront): same bars as
+    fp16x2;
   precision="fp32" (plain FP32: fp32 increments on MFMA, the reference's float32 wavefront
     typing): float32 storage of every cell bounds it; bar 1e-4 Frobenius / 1e-3 per entry.
 """
@@ -16,7 +18,7 @@ from helpers import brownian_np, rel_fro
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda"
-BARS = {"fp16x2": 1e-5, "fp32": 1e-4}
+BARS = {"fp16x2": 1e-5, "bf16x3": 1e-5, "fp32": 1e-4}
 PRECS = list(BARS)
 
 
