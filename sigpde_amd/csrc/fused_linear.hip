@@ -97,7 +97,7 @@ __global__ __launch_bounds__(256) void k_split_f32(const T *__restrict__ X, int 
 template <typename TO, int KB, int PREC>
 __global__ __launch_bounds__(64) void k_linear_gram_fused(GramFusedArgs<TO> a,
                                                           const double *sxd, const double *syd) {
-  using Prod = MfmaProducer<KB, true, PREC>;
+  using Prod = MfmaProducer<KB, 4, 1, PREC>;
   using E = typename Prod::elt;
   constexpr Arith A = PREC == kPrecFp32 ? Arith::Ref32 : Arith::Mixed;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -146,13 +146,13 @@ __global__ __launch_bounds__(64) void k_linear_gram_fused(GramFusedArgs<TO> a,
   for (int g = 0; g < 4; ++g) {
     const int64_t x = xq * 4 + g;
     P.xa[g] = x < a.Bx ? x : -1;
-    P.ybase[g] = y0;
   }
+  P.ybase[0] = y0;
+  P.ylane = y0;
   const int64_t xme = xq * 4 + lane / G;
   {
     const int64_t xc = xme < a.Bx ? xme : a.Bx - 1;
     P.xsl = PREC != kPrecFp32 ? a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0) : 1.0;
-    P.ysl = 1.0;
     P.pmY = 1.0;
   }
   ring16_solve<A>(P, geo, 1.0, smem_raw, [&](int sp, double r) {
@@ -291,7 +291,7 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
     while (ys > 1 && nq * ((By + ys - 1) / ys) < 4096) ys >>= 1;
     a.ys = ys;
     const int64_t nyr = (By + ys - 1) / ys;
-    // super-tiles of about one XCD's resident waves (32 CUs x 8)
+    // super-tiles of about one XCD's resident waves
     a.tx = (int)(nq < kTileX ? nq : kTileX);
     a.ty = (int)(nyr < kTileY ? nyr : kTileY);
     const int64_t nblk = (nq + a.tx - 1) / a.tx * a.tx * ((nyr + a.ty - 1) / a.ty * a.ty);

@@ -11,7 +11,7 @@ template <typename TO, int KB, int PREC>
 __global__ __launch_bounds__(64) void k_linear_pairwise_fused(GramFusedArgs<TO> a,
                                                               const double *sxd,
                                                               const double *syd) {
-  using Prod = MfmaProducer<KB, false, PREC>;
+  using Prod = MfmaProducer<KB, 4, 4, PREC>;
   using E = typename Prod::elt;
   constexpr Arith A = PREC == kPrecFp32 ? Arith::Ref32 : Arith::Mixed;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -43,14 +43,9 @@ __global__ __launch_bounds__(64) void k_linear_pairwise_fused(GramFusedArgs<TO> 
   const int64_t bme = (int64_t)blockIdx.x * 4 + lane / G;
   {
     const int64_t xc = bme < a.Bx ? bme : a.Bx - 1;
-    const int64_t yc = bme < a.By ? bme : a.By - 1;
-    if (PREC != kPrecFp32) {
-      P.xsl = a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0);
-      P.ysl = (double)a.invy[yc] * (syd ? syd[yc] : 1.0);
-    } else {
-      P.xsl = P.ysl = 1.0;
-    }
-    P.pmY = 1.0;
+    P.ylane = bme < a.By ? bme : a.By - 1;
+    P.xsl = PREC != kPrecFp32 ? a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0) : 1.0;
+    P.pmY = 1.0;  // the y scale is read when the pair's first chunk is multiplied
   }
   ring16_solve<A>(P, geo, 1.0, smem_raw, [&](int, double r) {
     if (bme < a.Bx) a.out[bme] = (TO)r;

@@ -23,7 +23,9 @@ constexpr int kPrecFp32 = 2;    // fp32 increments, v_mfma_f32_16x16x4f32
 constexpr int kPrecBf16x3 = 3;  // bf16 hi + mid + lo, 6 bf16 MFMAs per 32-wide K block
 
 // ------------------------------------------------------------------ MFMA producer (G = 16)
-// Lane group g (16 lanes) solves pairs (x path xa[g], y path ybase[g] + sp * ystep) of the stream.
+// Lane group g (16 lanes) solves pairs (x path xa[g], y path ybase[ys(g)] + sp * ystep) of the
+// stream: the wave holds four x-operand sets and NY y-operand sets (NY = 1: the Gram, four x paths
+// against one y path, whose operands all four tiles share; NY = 4: four independent pairs).
 // Chunk k of the stream covers 16 refined columns of one band (16 refined rows) of every group's
 // pair: one 16x16 MFMA tile per group, D[row][col] with A = the band's x-increment rows and
 // B = the chunk's y-increment columns (refined indices repeat original rows / columns 2^order
@@ -37,15 +39,17 @@ constexpr int kPrecBf16x3 = 3;  // bf16 hi + mid + lo, 6 bf16 MFMAs per 32-wide 
 //           K works as long as A and B use the same one); the tiles are scaled by
 //           kscale/4^order (x_scale y_scale) in fp32 and the consumer runs Ref32.
 // Pipeline of period p: at its start, multiply chunk p+1 and fetch chunk p+2's B operands; between
-// its consumer groups 1 and 2, write chunk p+1's tiles to the skewed ring of ring16.h (the
+// its consumer groups 2 and 3, write chunk p+1's tiles to the skewed ring of ring16.h (the
 // accumulators never live across periods).
 // After the last chunk of a band is multiplied, the next band's A operands are loaded. Operand
 // rows are addressed as a wave-uniform 64-bit row base plus a 32-bit per-lane offset.
-template <int KB, bool kSharedY, int PREC>
+template <int KB, int NX, int NY, int PREC>
 struct MfmaProducer {
   static constexpr int G = 16;
   static constexpr int NP = 4;           // pairs (lane groups) per wave
-  static constexpr int NB = kSharedY ? 1 : NP;
+  static_assert((NX == 4 && NY == 1) || (NX == 4 && NY == 4), "operand sets");
+  __device__ static constexpr int xs(int g) { return g; }
+  __device__ static constexpr int ys(int g) { return NY == 4 ? g : 0; }
   static constexpr bool kF16 = PREC == kPrecFp16x2;
   static constexpr bool kB3 = PREC == kPrecBf16x3;
   static constexpr bool kSplit = kF16 || kB3;     // 16-bit parts, pair scale applied by Mixed
@@ -58,30 +62,31 @@ struct MfmaProducer {
   const elt *xm, *ym;                    // bf16x3: middle parts
   int dpad;
   int m, n;                              // original increments per x / y path
-  int64_t xa[NP];                        // x path of each group (-1: idle)
-  int64_t ybase[NP];
-  int ystep;
+  int64_t xa[NX];                        // x path of each x set (-1: idle)
+  int64_t ybase[NY];                     // y path of each y set at stream position 0
+  int64_t ylane;                         // per lane: y path of the lane's pair at position 0
+  int ystep;                             // y path advance per stream position
   const float *invy;
   const double *sx, *sy;                 // optional path scales (robust Gram), may be null
   int64_t ny;                            // valid y paths (ybase + sp*ystep < ny)
   double kmult;                          // fp32: kscale / 4^order
   double xsl;                            // 16-bit splits, per lane: kscale 4^-order 2^-ex (x_scale)
-  double ysl;                            // 16-bit splits, pairwise, per lane: 2^-ey (y_scale)
-  double pmY;                            // 16-bit splits, Gram: 2^-ey (y_scale) of the pair multiplied last
+  double pmY;                            // 16-bit splits, per lane: 2^-ey (y_scale) of the lane's
+                                         // pair multiplied last
 
-  vec Ah[NP][NV], Al[NP][kSplit ? NV : 1], Am[NP][kB3 ? NV : 1];  // A operands of the band
-  vec Bh[NB][NV], Bl[NB][kSplit ? NV : 1], Bm[NB][kB3 ? NV : 1];  // next chunk's B operands
+  vec Ah[NX][NV], Al[NX][kSplit ? NV : 1], Am[NX][kB3 ? NV : 1];  // A operands of the band
+  vec Bh[NY][NV], Bl[NY][kSplit ? NV : 1], Bm[NY][kB3 ? NV : 1];  // next chunk's B operands
   float4v acc[NP];
   float mult[NP];                        // fp32: scale of the tiles in acc
   float pm[NP];                          // fp32: scale of the pair being multiplied
-  int64_t yrow[NB];                      // first B row (in elements) of the pair being fetched
+  int64_t yrow[NY];                      // first B row (in elements) of the pair being fetched
   Cursor<G> cB, cC;                      // next chunk to fetch B for / to multiply
   Ring16Writer<float> W;                 // chunk writes into the skewed ring
 
-  __device__ __forceinline__ double lane_scale() const { return kSharedY ? xsl * pmY : xsl * ysl; }
+  __device__ __forceinline__ double lane_scale() const { return xsl * pmY; }
 
-  __device__ __forceinline__ int64_t ypath(int g, int sp) const {
-    const int64_t yp = ybase[g] + (int64_t)sp * ystep;
+  __device__ __forceinline__ int64_t ypath(int q, int sp) const {
+    const int64_t yp = ybase[q] + (int64_t)sp * ystep;
     return yp < ny ? yp : ny - 1;
   }
   // per-lane element offset of this lane's K slice in an operand row
@@ -109,10 +114,10 @@ struct MfmaProducer {
     ro = ro > m - 1 ? m - 1 : ro;
     const uint32_t lo = (uint32_t)(ro * dpad) + kslice();
 #pragma unroll
-    for (int g = 0; g < NP; ++g) {
-      const int64_t xrow = (xa[g] < 0 ? 0 : xa[g]) * m * (int64_t)dpad;
-      load_row(xh + xrow, kB3 ? xm + xrow : nullptr, kSplit ? xl + xrow : nullptr, lo, Ah[g],
-               Am[g], Al[g]);
+    for (int a = 0; a < NX; ++a) {
+      const int64_t xrow = (xa[a] < 0 ? 0 : xa[a]) * m * (int64_t)dpad;
+      load_row(xh + xrow, kB3 ? xm + xrow : nullptr, kSplit ? xl + xrow : nullptr, lo, Ah[a],
+               Am[a], Al[a]);
     }
   }
   // B operands of chunk cB, then advance cB.
@@ -120,13 +125,13 @@ struct MfmaProducer {
     const int lane = threadIdx.x & 63;
     if (cB.band == 0 && cB.jv0 == 0) {
 #pragma unroll
-      for (int q = 0; q < NB; ++q) yrow[q] = ypath(q, cB.sp) * n * dpad;
+      for (int q = 0; q < NY; ++q) yrow[q] = ypath(q, cB.sp) * n * dpad;
     }
     int co = (cB.jv0 + (lane & 15) - 1) >> geo.order;
     co = co < 0 ? 0 : (co > n - 1 ? n - 1 : co);
     const uint32_t lo = (uint32_t)(co * dpad) + kslice();
 #pragma unroll
-    for (int q = 0; q < NB; ++q)
+    for (int q = 0; q < NY; ++q)
       load_row(yh + yrow[q], kB3 ? ym + yrow[q] : nullptr, kSplit ? yl + yrow[q] : nullptr, lo,
                Bh[q], Bm[q], Bl[q]);
     cB.advance(geo);
@@ -137,15 +142,15 @@ struct MfmaProducer {
   __device__ __forceinline__ void compute(const PairGeom<G> &geo, bool more) {
     if (cC.band == 0 && cC.jv0 == 0) {
       if constexpr (kSplit) {
-        if (kSharedY) {
-          const int64_t yp = ypath(0, cC.sp);
-          pmY = sy ? (double)invy[yp] * sy[yp] : (double)invy[yp];
-        }
+        // y scale of this lane's pair (a per-lane load)
+        int64_t yp = ylane + (int64_t)cC.sp * ystep;
+        yp = yp < ny ? yp : ny - 1;
+        pmY = sy ? (double)invy[yp] * sy[yp] : (double)invy[yp];
       } else {
 #pragma unroll
         for (int g = 0; g < NP; ++g) {
-          const int64_t xp = xa[g] < 0 ? 0 : xa[g];
-          const int64_t yp = ypath(g, cC.sp);
+          const int64_t xp = xa[xs(g)] < 0 ? 0 : xa[xs(g)];
+          const int64_t yp = ypath(ys(g), cC.sp);
           double mm = kmult;
           if (sx) mm *= sx[xp];
           if (sy) mm *= sy[yp];
@@ -155,42 +160,42 @@ struct MfmaProducer {
     }
 #pragma unroll
     for (int g = 0; g < NP; ++g) {
-      const int q = kSharedY ? 0 : g;
-      float4v a = {0.f, 0.f, 0.f, 0.f};
+      const int a = xs(g), q = ys(g);
+      float4v c = {0.f, 0.f, 0.f, 0.f};
       if constexpr (kF16) {
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) {
-          a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al[g][kb], Bh[q][kb], a, 0, 0, 0);
-          a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[g][kb], Bl[q][kb], a, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al[a][kb], Bh[q][kb], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[a][kb], Bl[q][kb], c, 0, 0, 0);
         }
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
-          a = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[g][kb], Bh[q][kb], a, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[a][kb], Bh[q][kb], c, 0, 0, 0);
       } else if constexpr (kB3) {
         // hi*lo + mid*mid + lo*hi (2^-16), then hi*mid + mid*hi (2^-8), then hi*hi
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) {
-          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[g][kb], Bl[q][kb], a, 0, 0, 0);
-          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am[g][kb], Bm[q][kb], a, 0, 0, 0);
-          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[g][kb], Bh[q][kb], a, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[a][kb], Bl[q][kb], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am[a][kb], Bm[q][kb], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[a][kb], Bh[q][kb], c, 0, 0, 0);
         }
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) {
-          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[g][kb], Bm[q][kb], a, 0, 0, 0);
-          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am[g][kb], Bh[q][kb], a, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[a][kb], Bm[q][kb], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am[a][kb], Bh[q][kb], c, 0, 0, 0);
         }
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
-          a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[g][kb], Bh[q][kb], a, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[a][kb], Bh[q][kb], c, 0, 0, 0);
       } else {
 #pragma unroll
         for (int v = 0; v < NV; ++v)
 #pragma unroll
           for (int e = 0; e < 4; ++e)
-            a = __builtin_amdgcn_mfma_f32_16x16x4f32(Ah[g][v][e], Bh[q][v][e], a, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x4f32(Ah[a][v][e], Bh[q][v][e], c, 0, 0, 0);
         mult[g] = pm[g];
       }
-      acc[g] = a;
+      acc[g] = c;
     }
     cC.advance(geo);
     if (more && cC.jv0 == 0) fetchA(geo, cC.band);
@@ -222,7 +227,7 @@ struct MfmaProducer {
     if (per + 1 < nchunks) compute(geo, per + 2 < nchunks);
     if (per + 2 < nchunks) fetchB(geo);
   }
-  // between consumer groups 1 and 2 of period p: write chunk p+1 (ring parity PAR)
+  // between consumer groups 2 and 3 of period p: write chunk p+1 (ring parity PAR)
   template <int PAR>
   __device__ __forceinline__ void period(float *ring, const PairGeom<G> &geo, int per) {
     const int nchunks = geo.columns() / kChunk;

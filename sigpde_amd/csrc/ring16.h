@@ -12,7 +12,7 @@
 //     At step t every lane reads slot t mod 32, element `lane`: one address per lane for the whole
 //     solve, the slot an immediate offset. Chunk k (16 columns) is written in period k-1, after
 //     the reads of that period were issued (its slots held chunk k-2, last read at step 16k-2).
-//     Ring and boundary inputs are loaded two groups (8 steps) before their use.
+//     Ring and boundary inputs are loaded one group (4 steps) before their use.
 //   * Period loop unrolled by two (even / odd parity), so ring slots are compile-time constants.
 //   * Band boundary line indexed by step: the group's last lane writes its value of step T at
 //     index T mod Np; the group's first lane reads, at step T, index (T + 15) mod Np, written at
@@ -26,6 +26,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "ring_solver.h"
 
 namespace sigpde {
@@ -33,12 +35,23 @@ namespace sigpde {
 constexpr int kR16Slots = 32;   // ring slots (skew 16 + one chunk of 16)
 constexpr int kR16Stride = 65;  // elements per slot: 64 rows + 1 pad (conflict-free chunk writes)
 
-// LDS for one wave: four boundary lines of Np + 16 doubles, then the ring.
+// Element type of the band boundary line: float where every cell is rounded to float anyway
+// (Ref32, Scaled32: exact), and for Mixed, whose band boundary rows are rounded to float (about
+// 1e-7 relative on the result at 255 x 255 grids, far below the fp32 MFMA increment error); the
+// float64 modes keep double.
+template <Arith A>
+struct R16Bnd {
+  using T = typename std::conditional<A == Arith::Ref64 || A == Arith::Scaled64, double,
+                                      float>::type;
+};
+
+// LDS for one wave: four boundary lines of Np + 16 elements, then the ring.
 template <Arith A>
 __host__ __device__ inline size_t r16_lds_bytes(int N) {
   using RT = typename ArithTraits<A>::RT;
+  using BT = typename R16Bnd<A>::T;
   const int np = Layout<16>::band_len(N);
-  return (size_t)4 * (np + 16) * sizeof(double) + (size_t)kR16Slots * kR16Stride * sizeof(RT);
+  return (size_t)4 * (np + 16) * sizeof(BT) + (size_t)kR16Slots * kR16Stride * sizeof(RT);
 }
 
 // Per-lane byte offsets of a producer's chunk writes. The producing lane l = (c, h), c = l & 15,
@@ -119,7 +132,7 @@ struct Global16Producer {
     if (g.columns() > kChunk) load(g);
   }
   __device__ __forceinline__ void early(RT *, const PairGeom<16> &, int) {}
-  // between consumer groups 1 and 2 of period per: write chunk per+1 (parity PAR), load per+2
+  // between consumer groups 2 and 3 of period per: write chunk per+1 (parity PAR), load per+2
   template <int PAR>
   __device__ __forceinline__ void period(RT *ring, const PairGeom<16> &g, int per) {
     const int nchunks = g.columns() / kChunk;
@@ -133,7 +146,7 @@ struct Global16Producer {
 template <Arith A>
 struct R16In {
   typename ArithTraits<A>::RT c[4];
-  double bv[4];
+  typename R16Bnd<A>::T bv[4];
 };
 
 template <Arith A>
@@ -154,8 +167,8 @@ using R16Ones = R16Flavour<false, true, false>;
 using R16Special = R16Flavour<true, false, true>;
 
 template <Arith A>
-__device__ __forceinline__ void r16_load(const char *rb, int slot0, const double *bp,
-                                         R16In<A> &in) {
+__device__ __forceinline__ void r16_load(const char *rb, int slot0,
+                                         const typename R16Bnd<A>::T *bp, R16In<A> &in) {
   using RT = typename ArithTraits<A>::RT;
 #pragma unroll
   for (int q = 0; q < 4; ++q)
@@ -165,38 +178,40 @@ __device__ __forceinline__ void r16_load(const char *rb, int slot0, const double
 }
 
 // Per-period pointers and flags.
+template <typename BT>
 struct R16Period {
-  const double *bvCur;   // boundary line + rst0 + 15 (reads of this period)
-  const double *bvNext;  // boundary line + rst0' + 15 (first group of the next period)
-  double *w;             // boundary line + rst0 (writes of the group's last lane)
-  double *wm;            // boundary line + Np + rst0 (mirror writes; reset periods only)
+  const BT *bvCur;   // boundary line + rst0 + 15 (reads of this period)
+  const BT *bvNext;  // boundary line + rst0' + 15 (first group of the next period)
+  BT *w;             // boundary line + rst0 (writes of the group's last lane)
+  BT *wm;            // boundary line + Np + rst0 (mirror writes; reset periods only)
   bool ones;             // Special: the values written are ones
   int sCap;              // Special: step of the result cell in this period, or -1
   int rsi;               // Special: step at which this lane resets (its rho), or -1
 };
 
-// Group I (4 steps) of a period with ring parity PAR (slots 16 PAR + s). Inputs are loaded two
-// groups ahead: group I loads group I+2, which for I >= 2 is group I-2 of the next period (ring
-// parity 1 - PAR; those loads follow the producer's write of chunk per+1, made between groups 1
-// and 2, after the reads of groups 2 and 3 of this period were issued).
+// Group I (4 steps) of a period with ring parity PAR (slots 16 PAR + s). Each group loads the
+// next group's inputs; group 3 loads the next period's group 0 (ring parity 1 - PAR), after the
+// producer has written chunk per+1 (between groups 2 and 3, once the reads of group 3 of this
+// period were issued).
 template <Arith A, int PAR, class F, int I>
-__device__ __forceinline__ void r16_group(const char *rb, const R16Period &pp, double s,
-                                          R16State<A> &st, R16In<A> (&q)[2], double &result) {
+__device__ __forceinline__ void r16_group(const char *rb, const R16Period<typename R16Bnd<A>::T> &pp,
+                                          double s, R16State<A> &st, R16In<A> &nxt,
+                                          double &result) {
+  using BT = typename R16Bnd<A>::T;
   const int lane = threadIdx.x & 63;
   const int rho = lane & 15;
-  R16In<A> cur = q[0];
-  q[0] = q[1];
-  if constexpr (I < 2)
-    r16_load<A>(rb, 16 * PAR + 4 * (I + 2), pp.bvCur + 4 * (I + 2), q[1]);
+  R16In<A> cur = nxt;
+  if constexpr (I < 3)
+    r16_load<A>(rb, 16 * PAR + 4 * (I + 1), pp.bvCur + 4 * (I + 1), nxt);
   else
-    r16_load<A>(rb, 16 * (1 - PAR) + 4 * (I - 2), pp.bvNext + 4 * (I - 2), q[1]);
-  double vals[4];
+    r16_load<A>(rb, 16 * (1 - PAR), pp.bvNext, nxt);
+  BT vals[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int sidx = 4 * I + k;
     double cA, cB;
     coefs<A>(cur.c[k], s, st.h, st.q, cA, cB);
-    const double up = shift_down<16>(st.k, cur.bv[k]);
+    const double up = shift_down<16>(st.k, (double)cur.bv[k]);
     double kn = cell<A>(up, st.k, st.kPrev, cA, cB);
     st.kPrev = up;
     if constexpr (F::kReset) {
@@ -211,17 +226,17 @@ __device__ __forceinline__ void r16_group(const char *rb, const R16Period &pp, d
     if constexpr (F::kCapture) {
       if (sidx == pp.sCap) result = kn;
     }
-    vals[k] = kn;
+    vals[k] = (BT)kn;
   }
   if (rho == 15) {
-    double *w = pp.w + 4 * I;
+    BT *w = pp.w + 4 * I;
     if constexpr (F::kOnes) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) w[k] = 1.0;
+      for (int k = 0; k < 4; ++k) w[k] = (BT)1;
     } else if constexpr (F::kReset) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const double v = pp.ones ? 1.0 : vals[k];
+        const BT v = pp.ones ? (BT)1 : vals[k];
         w[k] = v;
         if (pp.wm) pp.wm[4 * I + k] = v;
       }
@@ -232,11 +247,13 @@ __device__ __forceinline__ void r16_group(const char *rb, const R16Period &pp, d
   }
 }
 
-template <Arith A, int PAR, class F, bool kSecondHalf>
-__device__ __forceinline__ void r16_half(const char *rb, const R16Period &pp, double s,
-                                         R16State<A> &st, R16In<A> (&q)[2], double &result) {
-  r16_group<A, PAR, F, kSecondHalf ? 2 : 0>(rb, pp, s, st, q, result);
-  r16_group<A, PAR, F, kSecondHalf ? 3 : 1>(rb, pp, s, st, q, result);
+template <Arith A, int PAR, class F>
+__device__ __forceinline__ void r16_head(const char *rb, const R16Period<typename R16Bnd<A>::T> &pp,
+                                         double s, R16State<A> &st, R16In<A> &nxt,
+                                         double &result) {
+  r16_group<A, PAR, F, 0>(rb, pp, s, st, nxt, result);
+  r16_group<A, PAR, F, 1>(rb, pp, s, st, nxt, result);
+  r16_group<A, PAR, F, 2>(rb, pp, s, st, nxt, result);
 }
 
 // Solves the wave's stream (g.stream pairs per lane group, one after another). For every pair of
@@ -249,17 +266,18 @@ __device__ __forceinline__ void r16_half(const char *rb, const R16Period &pp, do
 template <Arith A, class Producer, class Out>
 __device__ void ring16_solve(Producer &P, const PairGeom<16> &g, double s, void *lds, Out &&out) {
   using RT = typename ArithTraits<A>::RT;
+  using BT = typename R16Bnd<A>::T;
   const int lane = threadIdx.x & 63;
   const int rho = lane & 15;
   const int Np = g.Np;
-  const int bs = Np + 16;
-  double *bnd = reinterpret_cast<double *>(lds);
+  const int bs = Np + 16;  // a multiple of 16: the ring after the lines stays 64-byte aligned
+  BT *bnd = reinterpret_cast<BT *>(lds);
   RT *ring = reinterpret_cast<RT *>(bnd + 4 * bs);
-  double *line = bnd + (lane >> 4) * bs;
+  BT *line = bnd + (lane >> 4) * bs;
   const char *rb = reinterpret_cast<const char *>(ring + lane);
 
   for (int i = lane; i < kR16Slots * kR16Stride; i += 64) ring[i] = (RT)0;
-  for (int i = lane; i < 4 * bs; i += 64) bnd[i] = 1.0;
+  for (int i = lane; i < 4 * bs; i += 64) bnd[i] = (BT)1;
 
   const int total = g.columns();
   const int nperiods = total / kChunk + 1;  // the last lane finishes at step total + 14
@@ -274,16 +292,15 @@ __device__ void ring16_solve(Producer &P, const PairGeom<16> &g, double s, void 
   R16State<A> st;
   st.k = st.kPrev = 1.0;
   st.h = st.q = st.hN = st.qN = 0.0;
-  R16In<A> q[2];  // inputs of the next two groups
-  r16_load<A>(rb, 0, line + 15, q[0]);
-  r16_load<A>(rb, 4, line + 19, q[1]);
+  R16In<A> nxt;  // inputs of the next group
+  r16_load<A>(rb, 0, line + 15, nxt);
 
   int rst0 = 0;   // t0 mod Np
   int band = 0;   // band of column t0 within its pair
   for (int per = 0; per < nperiods; ++per) {
     const bool reset = rst0 == 0;
     const int rn = rst0 + kChunk == Np ? 0 : rst0 + kChunk;
-    R16Period pp;
+    R16Period<BT> pp;
     pp.bvCur = line + rst0 + 15;
     pp.bvNext = line + rn + 15;
     pp.w = line + rst0;
@@ -304,30 +321,30 @@ __device__ void ring16_solve(Producer &P, const PairGeom<16> &g, double s, void 
     const bool odd = per & 1;
     const int fl = (reset || cap) ? 2 : (pp.ones ? 1 : 0);
     P.early(ring, g, per);
-    // groups 0, 1
+    // groups 0..2
     if (fl == 2) {
-      if (odd) r16_half<A, 1, R16Special, false>(rb, pp, s, st, q, result);
-      else r16_half<A, 0, R16Special, false>(rb, pp, s, st, q, result);
+      if (odd) r16_head<A, 1, R16Special>(rb, pp, s, st, nxt, result);
+      else r16_head<A, 0, R16Special>(rb, pp, s, st, nxt, result);
     } else if (fl == 1) {
-      if (odd) r16_half<A, 1, R16Ones, false>(rb, pp, s, st, q, result);
-      else r16_half<A, 0, R16Ones, false>(rb, pp, s, st, q, result);
+      if (odd) r16_head<A, 1, R16Ones>(rb, pp, s, st, nxt, result);
+      else r16_head<A, 0, R16Ones>(rb, pp, s, st, nxt, result);
     } else {
-      if (odd) r16_half<A, 1, R16Plain, false>(rb, pp, s, st, q, result);
-      else r16_half<A, 0, R16Plain, false>(rb, pp, s, st, q, result);
+      if (odd) r16_head<A, 1, R16Plain>(rb, pp, s, st, nxt, result);
+      else r16_head<A, 0, R16Plain>(rb, pp, s, st, nxt, result);
     }
-    // chunk per+1 (parity of per+1) goes to the ring after the reads of groups 2, 3 were issued
+    // chunk per+1 (parity of per+1) goes to the ring after the reads of group 3 were issued
     if (odd) P.template period<0>(ring, g, per);
     else P.template period<1>(ring, g, per);
-    // groups 2, 3
+    // group 3
     if (fl == 2) {
-      if (odd) r16_half<A, 1, R16Special, true>(rb, pp, s, st, q, result);
-      else r16_half<A, 0, R16Special, true>(rb, pp, s, st, q, result);
+      if (odd) r16_group<A, 1, R16Special, 3>(rb, pp, s, st, nxt, result);
+      else r16_group<A, 0, R16Special, 3>(rb, pp, s, st, nxt, result);
     } else if (fl == 1) {
-      if (odd) r16_half<A, 1, R16Ones, true>(rb, pp, s, st, q, result);
-      else r16_half<A, 0, R16Ones, true>(rb, pp, s, st, q, result);
+      if (odd) r16_group<A, 1, R16Ones, 3>(rb, pp, s, st, nxt, result);
+      else r16_group<A, 0, R16Ones, 3>(rb, pp, s, st, nxt, result);
     } else {
-      if (odd) r16_half<A, 1, R16Plain, true>(rb, pp, s, st, q, result);
-      else r16_half<A, 0, R16Plain, true>(rb, pp, s, st, q, result);
+      if (odd) r16_group<A, 1, R16Plain, 3>(rb, pp, s, st, nxt, result);
+      else r16_group<A, 0, R16Plain, 3>(rb, pp, s, st, nxt, result);
     }
     if (cap) {
       if (rho == rM) out(sp, result);
