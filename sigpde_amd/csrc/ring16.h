@@ -33,7 +33,7 @@
 namespace sigpde {
 
 constexpr int kR16Slots = 32;   // ring slots (skew 16 + one chunk of 16)
-constexpr int kR16Stride = 65;  // elements per slot: 64 rows + 1 pad (conflict-free chunk writes)
+constexpr int kR16Stride = 67;  // elements per slot: 64 rows + 3 pad (conflict-free chunk writes)
 
 // Element type of the band boundary line: float where every cell is rounded to float anyway
 // (Ref32, Scaled32: exact), and for Mixed, whose band boundary rows are rounded to float (about
@@ -45,13 +45,22 @@ struct R16Bnd {
                                       float>::type;
 };
 
-// LDS for one wave: four boundary lines of Np + 16 elements, then the ring.
+// Boundary line stride (elements): Np + 16 (mirror) plus a pad that puts the four lines of a wave
+// in different LDS banks (the lane groups read and write them in the same instructions).
+template <Arith A>
+__host__ __device__ inline int r16_line_stride(int Np) {
+  return Np + 16 + (sizeof(typename R16Bnd<A>::T) == 4 ? 8 : 4);
+}
+
+// LDS for one wave: four boundary lines, then the ring (the lines' total is a multiple of 128
+// bytes, so the ring stays aligned).
 template <Arith A>
 __host__ __device__ inline size_t r16_lds_bytes(int N) {
   using RT = typename ArithTraits<A>::RT;
   using BT = typename R16Bnd<A>::T;
   const int np = Layout<16>::band_len(N);
-  return (size_t)4 * (np + 16) * sizeof(BT) + (size_t)kR16Slots * kR16Stride * sizeof(RT);
+  return (size_t)4 * r16_line_stride<A>(np) * sizeof(BT) +
+         (size_t)kR16Slots * kR16Stride * sizeof(RT);
 }
 
 // Per-lane byte offsets of a producer's chunk writes. The producing lane l = (c, h), c = l & 15,
@@ -270,7 +279,7 @@ __device__ void ring16_solve(Producer &P, const PairGeom<16> &g, double s, void 
   const int lane = threadIdx.x & 63;
   const int rho = lane & 15;
   const int Np = g.Np;
-  const int bs = Np + 16;  // a multiple of 16: the ring after the lines stays 64-byte aligned
+  const int bs = r16_line_stride<A>(Np);
   BT *bnd = reinterpret_cast<BT *>(lds);
   RT *ring = reinterpret_cast<RT *>(bnd + 4 * bs);
   BT *line = bnd + (lane >> 4) * bs;
