@@ -47,6 +47,10 @@ MI355X_FP64_VALU_TFLOPS = 78.6
 MI355X_FP16_MFMA_TFLOPS = 2500.0
 MI355X_FP32_MFMA_TFLOPS = 157.3
 FLOPS_PER_CELL = 10  # pde_step as written
+# HBM bytes per launch of the fused fp16x2 Gram at the default workload (1024 x 1024 paths, length
+# 256, dim 64, order 0), from rocprofv3 FETCH_SIZE + WRITE_SIZE passes scaled by a 256 MiB copy
+# (profiles/gram_fp16x2_1024x1024_L256_d64_o0_traffic.md); other workloads report null.
+MEASURED_TRAFFIC = {("fp16x2", 1024, 256, 64, 0): 8.443e9}
 
 
 def parse():
@@ -124,6 +128,10 @@ def fused_roofline(args, X, Y, ms):
                       "wavefront, one launch)",
             "fp32": "k_linear_gram_fused<float, 2, 2> (fp32 MFMA increments + float32-typed "
                     "wavefront, one launch)"}[args.mode]
+    traffic = MEASURED_TRAFFIC.get((args.mode, args.paths, args.length, args.dim, args.order))
+    alg_bytes = (X.shape[0] * (X.shape[1] - 1) + Y.shape[0] * (Y.shape[1] - 1)) * \
+        ((args.dim + 31) // 32 * 32) * {"fp16x2": 4, "bf16x3": 6, "fp32": 4}[args.mode] + \
+        pairs * X.element_size()
     return {
         "kernel": name,
         "bound": "valu",
@@ -131,14 +139,19 @@ def fused_roofline(args, X, Y, ms):
         "peak": MI355X_FP64_VALU_TFLOPS,
         "unit": "TFLOP/s",
         "frac": flops_wave / dur_s / 1e12 / MI355X_FP64_VALU_TFLOPS,
-        "traffic": None,
+        "traffic": traffic,
+        "algorithmic_bytes": alg_bytes,
+        "hbm_gbs": None if traffic is None else traffic / dur_s / 1e9,
         "launch_ms": dur_s * 1e3,
         "cells_per_s": cells / dur_s,
         "increment_gram_tflops": flops_inc / dur_s / 1e12,
         "increment_gram_mfma_frac": flops_inc / dur_s / 1e12 / (
             MI355X_FP32_MFMA_TFLOPS if args.mode == "fp32" else MI355X_FP16_MFMA_TFLOPS),
-        "note": "wavefront flops = 10 per refined cell (pde_step as written); the increment Gram "
-                "(2*d flops per increment) runs on MFMA beside it",
+        "note": "bound: the wavefront's FP64 VALU (the cell update is 8 FP64 instructions; peak = "
+                "MI355X FP64 vector 78.6 TFLOP/s, which is also the FP32 rate without packed math); "
+                "wavefront flops = 10 per refined cell (pde_step as written); the increment Gram "
+                "(2*d flops per increment) runs on MFMA beside it; traffic = measured HBM bytes per "
+                "launch, algorithmic_bytes = operands + output",
     }
 
 
