@@ -94,8 +94,12 @@ __global__ __launch_bounds__(256) void k_split_f32(const T *__restrict__ X, int 
 // Gram: a wave takes four x paths (a quad) and streams `ys` y paths through them.
 // Symmetric: only y <= x pairs are written, with their mirror.
 
+// Occupancy: at dim <= 32 (one K block) the fp16x2 kernel fits three waves per SIMD (measured 9%
+// faster at dim 16); the other instantiations would spill there and keep the compiler's choice.
 template <typename TO, int KB, int PREC>
-__global__ __launch_bounds__(64) void k_linear_gram_fused(GramFusedArgs<TO> a,
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(KB == 1 && PREC == kPrecFp16x2 ? 3 : 1))) void
+k_linear_gram_fused(GramFusedArgs<TO> a,
                                                           const double *sxd, const double *syd) {
   using Prod = MfmaProducer<KB, 4, 1, PREC>;
   using E = typename Prod::elt;

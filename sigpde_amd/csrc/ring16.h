@@ -63,32 +63,35 @@ __host__ __device__ inline size_t r16_lds_bytes(int N) {
          (size_t)kR16Slots * kR16Stride * sizeof(RT);
 }
 
-// Per-lane byte offsets of a producer's chunk writes. The producing lane l = (c, h), c = l & 15,
-// h = l >> 4, holds element e (0..3) of pair g: ring row 16g + 4h + e, chunk column c. Chunk k
-// goes to slot (16 (k & 1) + c + 4h + e) & 31.
+// Chunk writes of a producer. The producing lane l = (c, h), c = l & 15, h = l >> 4, holds element
+// e (0..3) of pair g: ring row 16g + 4h + e, chunk column c. Chunk k goes to slot
+// (16 (k & 1) + c + 4h + e) & 31. Consecutive e are consecutive slots, so one base offset per
+// parity serves all four elements (immediate offsets e * (stride + 1)); only odd chunks can wrap
+// within a lane's four elements (when (16 + c + 4h) & 31 >= 29), corrected by one select each.
 template <typename RT>
 struct Ring16Writer {
-  uint32_t off[2][4];
+  uint32_t base[2];  // byte offset of element 0 (pair 0) for even / odd chunks
+  int s1;            // (16 + c + 4h) & 31: the odd chunks' first slot
   __device__ __forceinline__ void init() {
     const int lane = threadIdx.x & 63;
     const int c = lane & 15, h = lane >> 4;
-#pragma unroll
-    for (int p = 0; p < 2; ++p)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int slot = (16 * p + c + 4 * h + e) & 31;
-        off[p][e] = (uint32_t)((slot * kR16Stride + 4 * h + e) * sizeof(RT));
-      }
+    const int s0 = c + 4 * h;  // <= 27: even chunks never wrap
+    s1 = (16 + c + 4 * h) & 31;
+    base[0] = (uint32_t)((s0 * kR16Stride + 4 * h) * sizeof(RT));
+    base[1] = (uint32_t)((s1 * kR16Stride + 4 * h) * sizeof(RT));
   }
   // v[g][e]: the lane's four values of pair g
   template <int PAR>
   __device__ __forceinline__ void store(RT *ring, const RT (&v)[4][4]) const {
-    char *base = reinterpret_cast<char *>(ring);
+    char *rbase = reinterpret_cast<char *>(ring);
+    constexpr uint32_t kWrap = (uint32_t)(kR16Slots * kR16Stride * sizeof(RT));
 #pragma unroll
-    for (int g = 0; g < 4; ++g)
+    for (int e = 0; e < 4; ++e) {
+      uint32_t a = base[PAR] + (uint32_t)(e * (kR16Stride + 1) * sizeof(RT));
+      if (PAR == 1 && e > 0) a = s1 + e >= kR16Slots ? a - kWrap : a;
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        *reinterpret_cast<RT *>(base + off[PAR][e] + g * 16 * sizeof(RT)) = v[g][e];
+      for (int g = 0; g < 4; ++g) *reinterpret_cast<RT *>(rbase + a + g * 16 * sizeof(RT)) = v[g][e];
+    }
   }
 };
 
