@@ -96,9 +96,9 @@ __global__ __launch_bounds__(256) void k_split_f32(const T *__restrict__ X, int 
 
 // Occupancy: at dim <= 32 (one K block) the fp16x2 kernel fits three waves per SIMD (measured 9%
 // faster at dim 16); the other instantiations would spill there and keep the compiler's choice.
-template <typename TO, int KB, int PREC>
+template <typename TO, int KB, int PREC, int R>
 __global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(KB == 1 && PREC == kPrecFp16x2 ? 3 : 1))) void
+__attribute__((amdgpu_waves_per_eu(KB == 1 && PREC == kPrecFp16x2 && R == 1 ? 3 : 1))) void
 k_linear_gram_fused(GramFusedArgs<TO> a,
                                                           const double *sxd, const double *syd) {
   using Prod = MfmaProducer<KB, 4, 1, PREC>;
@@ -129,7 +129,7 @@ k_linear_gram_fused(GramFusedArgs<TO> a,
     if (y0 > xmax) return;
     if (y0 + ys - 1 > xmax) ys = (int)(xmax - y0 + 1);
   }
-  PairGeom<G> geo(a.m, a.n, a.order, ys);
+  PairGeom<G> geo(a.m, a.n, a.order, ys, R);
   Prod P;
   P.xh = (const E *)a.xh;
   P.xl = (const E *)a.xl;
@@ -159,7 +159,7 @@ k_linear_gram_fused(GramFusedArgs<TO> a,
     P.xsl = PREC != kPrecFp32 ? a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0) : 1.0;
     P.pmY = 1.0;
   }
-  ring16_solve<A>(P, geo, 1.0, smem_raw, [&](int sp, double r) {
+  ring16_solve<A, R>(P, geo, 1.0, smem_raw, [&](int sp, double r) {
     const int64_t y = y0 + sp;
     if (xme >= a.Bx || y >= a.By) return;
     if (a.symmetric) {
@@ -301,7 +301,13 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
     const int64_t nblk = (nq + a.tx - 1) / a.tx * a.tx * ((nyr + a.ty - 1) / a.ty * a.ty);
     // a multiple of 8 blocks so that the XCD-major block order covers every work unit
     const int64_t grid = (nblk + 7) / 8 * 8;
-    k_linear_gram_fused<T, KB, PREC><<<dim3((unsigned)grid), dim3(64), shmem, st>>>(a, sxd, syd);
+    const dim3 g3((unsigned)grid), b3(64);
+    switch (r16_rows(order)) {
+      case 1: k_linear_gram_fused<T, KB, PREC, 1><<<g3, b3, shmem, st>>>(a, sxd, syd); break;
+      case 2: k_linear_gram_fused<T, KB, PREC, 2><<<g3, b3, shmem, st>>>(a, sxd, syd); break;
+      case 4: k_linear_gram_fused<T, KB, PREC, 4><<<g3, b3, shmem, st>>>(a, sxd, syd); break;
+      default: k_linear_gram_fused<T, KB, PREC, 8><<<g3, b3, shmem, st>>>(a, sxd, syd); break;
+    }
   }
   return hipGetLastError();
 }

@@ -7,7 +7,7 @@
 namespace sigpde {
 
 // Pairwise: a wave solves four pairs (x_b, y_b).
-template <typename TO, int KB, int PREC>
+template <typename TO, int KB, int PREC, int R>
 __global__ __launch_bounds__(64) void k_linear_pairwise_fused(GramFusedArgs<TO> a,
                                                               const double *sxd,
                                                               const double *syd) {
@@ -17,7 +17,7 @@ __global__ __launch_bounds__(64) void k_linear_pairwise_fused(GramFusedArgs<TO> 
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int G = 16;
   const int lane = threadIdx.x & 63;
-  PairGeom<G> geo(a.m, a.n, a.order, 1);
+  PairGeom<G> geo(a.m, a.n, a.order, 1, R);
   Prod P;
   P.xh = (const E *)a.xh;
   P.xl = (const E *)a.xl;
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(64) void k_linear_pairwise_fused(GramFusedArgs<TO> 
     P.xsl = PREC != kPrecFp32 ? a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0) : 1.0;
     P.pmY = 1.0;  // the y scale is read when the pair's first chunk is multiplied
   }
-  ring16_solve<A>(P, geo, 1.0, smem_raw, [&](int, double r) {
+  ring16_solve<A, R>(P, geo, 1.0, smem_raw, [&](int, double r) {
     if (bme < a.Bx) a.out[bme] = (TO)r;
   });
 }
@@ -55,8 +55,13 @@ __global__ __launch_bounds__(64) void k_linear_pairwise_fused(GramFusedArgs<TO> 
 template <typename T, int KB, int PREC>
 hipError_t launch_linear_pairwise_fused(const GramFusedArgs<T> &a, const double *sxd,
                                         const double *syd, size_t shmem, hipStream_t st) {
-  const unsigned blocks = (unsigned)((a.Bx + 3) / 4);
-  k_linear_pairwise_fused<T, KB, PREC><<<dim3(blocks), dim3(64), shmem, st>>>(a, sxd, syd);
+  const dim3 grid((unsigned)((a.Bx + 3) / 4)), block(64);
+  switch (r16_rows(a.order)) {
+    case 1: k_linear_pairwise_fused<T, KB, PREC, 1><<<grid, block, shmem, st>>>(a, sxd, syd); break;
+    case 2: k_linear_pairwise_fused<T, KB, PREC, 2><<<grid, block, shmem, st>>>(a, sxd, syd); break;
+    case 4: k_linear_pairwise_fused<T, KB, PREC, 4><<<grid, block, shmem, st>>>(a, sxd, syd); break;
+    default: k_linear_pairwise_fused<T, KB, PREC, 8><<<grid, block, shmem, st>>>(a, sxd, syd); break;
+  }
   return hipGetLastError();
 }
 

@@ -110,7 +110,7 @@ struct MfmaProducer {
 
   __device__ __forceinline__ void fetchA(const PairGeom<G> &geo, int band) {
     const int lane = threadIdx.x & 63;
-    int ro = (band * G + (lane & 15)) >> geo.order;
+    int ro = (band * G * geo.R + geo.R * (lane & 15)) >> geo.order;
     ro = ro > m - 1 ? m - 1 : ro;
     const uint32_t lo = (uint32_t)(ro * dpad) + kslice();
 #pragma unroll

@@ -21,6 +21,11 @@
 //   * Three period flavours chosen once per period: Plain, Ones (the boundary values written are
 //     ones: the group's last lane is in a pair's last band, whose next band is the next pair's
 //     first), Special (a reset column, where k = 1, and/or the result cell).
+//   * R refined rows per lane (R = min(2^order, 8)): lane rho owns rows R rho .. R rho + R - 1 of a
+//     band of 16 R rows; at dyadic order >= log2 R these rows refine one original row, so at every
+//     column all R cells share one increment: the coefficients and the DPP shift are paid once per
+//     R cells. The lane's rows are updated top to bottom within a step (row r takes row r-1's new
+//     value from the same lane); the ring holds one increment per lane and column, as for R = 1.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,6 +38,9 @@
 namespace sigpde {
 
 constexpr int kR16Slots = 32;   // ring slots (skew 16 + one chunk of 16)
+
+// Refined rows per lane at dyadic order o: 2^o, at most 8.
+__host__ __device__ inline int r16_rows(int order) { return order >= 3 ? 8 : 1 << order; }
 constexpr int kR16Stride = 67;  // elements per slot: 64 rows + 3 pad (conflict-free chunk writes)
 
 // Element type of the band boundary line: float where every cell is rounded to float anyway
@@ -97,8 +105,9 @@ struct Ring16Writer {
 
 // Producer of increments read from global memory (any static kernel; torch computed them).
 // Group g serves the pair whose increments (m x n, row stride ld) start at inc[g] (null: an idle
-// slot, served zeros). Lane (c, h) loads element e of pair g = ring row 16g + 4h + e (refined row
-// 16 band + 4h + e) at chunk column c: four 64-byte row segments per load. Chunk k is loaded two
+// slot, served zeros). Lane (c, h) loads element e of pair g = ring row 16g + 4h + e (the lane
+// 4h + e of that group: refined rows R (16 band + 4h + e) ..) at chunk column c: four 64-byte
+// row segments per load. Chunk k is loaded two
 // periods before its use and scaled when stored.
 template <typename RT>
 struct Global16Producer {
@@ -117,7 +126,7 @@ struct Global16Producer {
     jo = jo < 0 ? 0 : (jo > g.n - 1 ? g.n - 1 : jo);
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      int ro = (cl.band * 16 + 4 * h + e) >> g.order;
+      int ro = (cl.band * 16 * g.R + g.R * (4 * h + e)) >> g.order;
       ro = ro > g.m - 1 ? g.m - 1 : ro;
 #pragma unroll
       for (int p = 0; p < 4; ++p) {
@@ -161,9 +170,10 @@ struct R16In {
   typename R16Bnd<A>::T bv[4];
 };
 
-template <Arith A>
+template <Arith A, int R>
 struct R16State {
-  double k, kPrev;
+  double k[R];    // the lane's rows at its current column
+  double kPrev;   // the row above the lane's first row, at the previous column
   double h, q;    // Mixed: coefficients of the lane's current pair
   double hN, qN;  // Mixed: coefficients taken at the lane's next reset
 };
@@ -205,9 +215,9 @@ struct R16Period {
 // next group's inputs; group 3 loads the next period's group 0 (ring parity 1 - PAR), after the
 // producer has written chunk per+1 (between groups 2 and 3, once the reads of group 3 of this
 // period were issued).
-template <Arith A, int PAR, class F, int I>
+template <Arith A, int R, int PAR, class F, int I>
 __device__ __forceinline__ void r16_group(const char *rb, const R16Period<typename R16Bnd<A>::T> &pp,
-                                          double s, R16State<A> &st, R16In<A> &nxt,
+                                          double s, R16State<A, R> &st, R16In<A> &nxt,
                                           double &result) {
   using BT = typename R16Bnd<A>::T;
   const int lane = threadIdx.x & 63;
@@ -223,22 +233,30 @@ __device__ __forceinline__ void r16_group(const char *rb, const R16Period<typena
     const int sidx = 4 * I + k;
     double cA, cB;
     coefs<A>(cur.c[k], s, st.h, st.q, cA, cB);
-    const double up = shift_down<16>(st.k, (double)cur.bv[k]);
-    double kn = cell<A>(up, st.k, st.kPrev, cA, cB);
+    const double up = shift_down<16>(st.k[R - 1], (double)cur.bv[k]);
+    bool z = false;
+    if constexpr (F::kReset) z = pp.rsi == sidx;
+    double above = up, aboveOld = st.kPrev;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double old = st.k[r];
+      double kn = cell<A>(above, old, aboveOld, cA, cB);
+      if constexpr (F::kReset) kn = z ? 1.0 : kn;
+      st.k[r] = kn;
+      aboveOld = old;
+      above = kn;
+    }
     st.kPrev = up;
     if constexpr (F::kReset) {
-      const bool z = pp.rsi == sidx;
-      kn = z ? 1.0 : kn;
       if constexpr (A == Arith::Mixed) {
         st.h = z ? st.hN : st.h;
         st.q = z ? st.qN : st.q;
       }
     }
-    st.k = kn;
     if constexpr (F::kCapture) {
-      if (sidx == pp.sCap) result = kn;
+      if (sidx == pp.sCap) result = st.k[R - 1];
     }
-    vals[k] = (BT)kn;
+    vals[k] = (BT)st.k[R - 1];
   }
   if (rho == 15) {
     BT *w = pp.w + 4 * I;
@@ -259,23 +277,24 @@ __device__ __forceinline__ void r16_group(const char *rb, const R16Period<typena
   }
 }
 
-template <Arith A, int PAR, class F>
+template <Arith A, int R, int PAR, class F>
 __device__ __forceinline__ void r16_head(const char *rb, const R16Period<typename R16Bnd<A>::T> &pp,
-                                         double s, R16State<A> &st, R16In<A> &nxt,
+                                         double s, R16State<A, R> &st, R16In<A> &nxt,
                                          double &result) {
-  r16_group<A, PAR, F, 0>(rb, pp, s, st, nxt, result);
-  r16_group<A, PAR, F, 1>(rb, pp, s, st, nxt, result);
-  r16_group<A, PAR, F, 2>(rb, pp, s, st, nxt, result);
+  r16_group<A, R, PAR, F, 0>(rb, pp, s, st, nxt, result);
+  r16_group<A, R, PAR, F, 1>(rb, pp, s, st, nxt, result);
+  r16_group<A, R, PAR, F, 2>(rb, pp, s, st, nxt, result);
 }
 
 // Solves the wave's stream (g.stream pairs per lane group, one after another). For every pair of
-// the stream, out(sp, value) is called by the lanes rho == (M-1) mod 16 with that group's k[M][N].
+// the stream, out(sp, value) is called by the lanes holding row M-1 (lane ((M-1) mod 16R) / R, its
+// last row) with that group's k[M][N]. g.R must equal R.
 //   P: producer (Global16Producer or the MFMA producer): prologue / early / period<PAR> /
 //      lane_scale (Arith::Mixed: the scale of the pair whose chunk was multiplied last, read at
 //      the start of the period that holds that pair's first reset)
 //   s: per-lane double scale of the Scaled* modes (ignored otherwise)
 //   lds: r16_lds_bytes<A>(N) bytes of this wave's LDS
-template <Arith A, class Producer, class Out>
+template <Arith A, int R, class Producer, class Out>
 __device__ void ring16_solve(Producer &P, const PairGeom<16> &g, double s, void *lds, Out &&out) {
   using RT = typename ArithTraits<A>::RT;
   using BT = typename R16Bnd<A>::T;
@@ -294,15 +313,17 @@ __device__ void ring16_solve(Producer &P, const PairGeom<16> &g, double s, void 
   const int total = g.columns();
   const int nperiods = total / kChunk + 1;  // the last lane finishes at step total + 14
   const int pc = g.pair_columns();
-  const int bM = (g.M - 1) >> 4;
-  const int rM = (g.M - 1) & 15;
+  const int bM = (g.M - 1) / (16 * R);
+  const int rM = ((g.M - 1) - bM * 16 * R) / R;
   int tM = bM * Np + g.N + rM;  // result step of stream pair sp
   int sp = 0;
 
   P.prologue(ring, g);
 
-  R16State<A> st;
-  st.k = st.kPrev = 1.0;
+  R16State<A, R> st;
+#pragma unroll
+  for (int r = 0; r < R; ++r) st.k[r] = 1.0;
+  st.kPrev = 1.0;
   st.h = st.q = st.hN = st.qN = 0.0;
   R16In<A> nxt;  // inputs of the next group
   r16_load<A>(rb, 0, line + 15, nxt);
@@ -335,28 +356,28 @@ __device__ void ring16_solve(Producer &P, const PairGeom<16> &g, double s, void 
     P.early(ring, g, per);
     // groups 0..2
     if (fl == 2) {
-      if (odd) r16_head<A, 1, R16Special>(rb, pp, s, st, nxt, result);
-      else r16_head<A, 0, R16Special>(rb, pp, s, st, nxt, result);
+      if (odd) r16_head<A, R, 1, R16Special>(rb, pp, s, st, nxt, result);
+      else r16_head<A, R, 0, R16Special>(rb, pp, s, st, nxt, result);
     } else if (fl == 1) {
-      if (odd) r16_head<A, 1, R16Ones>(rb, pp, s, st, nxt, result);
-      else r16_head<A, 0, R16Ones>(rb, pp, s, st, nxt, result);
+      if (odd) r16_head<A, R, 1, R16Ones>(rb, pp, s, st, nxt, result);
+      else r16_head<A, R, 0, R16Ones>(rb, pp, s, st, nxt, result);
     } else {
-      if (odd) r16_head<A, 1, R16Plain>(rb, pp, s, st, nxt, result);
-      else r16_head<A, 0, R16Plain>(rb, pp, s, st, nxt, result);
+      if (odd) r16_head<A, R, 1, R16Plain>(rb, pp, s, st, nxt, result);
+      else r16_head<A, R, 0, R16Plain>(rb, pp, s, st, nxt, result);
     }
     // chunk per+1 (parity of per+1) goes to the ring after the reads of group 3 were issued
     if (odd) P.template period<0>(ring, g, per);
     else P.template period<1>(ring, g, per);
     // group 3
     if (fl == 2) {
-      if (odd) r16_group<A, 1, R16Special, 3>(rb, pp, s, st, nxt, result);
-      else r16_group<A, 0, R16Special, 3>(rb, pp, s, st, nxt, result);
+      if (odd) r16_group<A, R, 1, R16Special, 3>(rb, pp, s, st, nxt, result);
+      else r16_group<A, R, 0, R16Special, 3>(rb, pp, s, st, nxt, result);
     } else if (fl == 1) {
-      if (odd) r16_group<A, 1, R16Ones, 3>(rb, pp, s, st, nxt, result);
-      else r16_group<A, 0, R16Ones, 3>(rb, pp, s, st, nxt, result);
+      if (odd) r16_group<A, R, 1, R16Ones, 3>(rb, pp, s, st, nxt, result);
+      else r16_group<A, R, 0, R16Ones, 3>(rb, pp, s, st, nxt, result);
     } else {
-      if (odd) r16_group<A, 1, R16Plain, 3>(rb, pp, s, st, nxt, result);
-      else r16_group<A, 0, R16Plain, 3>(rb, pp, s, st, nxt, result);
+      if (odd) r16_group<A, R, 1, R16Plain, 3>(rb, pp, s, st, nxt, result);
+      else r16_group<A, R, 0, R16Plain, 3>(rb, pp, s, st, nxt, result);
     }
     if (cap) {
       if (rho == rM) out(sp, result);

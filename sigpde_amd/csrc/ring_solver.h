@@ -161,14 +161,15 @@ struct PairGeom {
   int m, n, order;  // original increments and dyadic order
   int M, N;         // refined rows / columns
   int Np;           // virtual columns per band
-  int nb;           // bands of G rows
+  int R;            // refined rows per lane (a power of two dividing 2^order; ring16.h)
+  int nb;           // bands of G * R rows
   int stream;       // pairs solved one after another
-  __device__ __host__ PairGeom(int m_, int n_, int o_, int stream_ = 1)
-      : m(m_), n(n_), order(o_), stream(stream_) {
+  __device__ __host__ PairGeom(int m_, int n_, int o_, int stream_ = 1, int R_ = 1)
+      : m(m_), n(n_), order(o_), R(R_), stream(stream_) {
     M = m << o_;
     N = n << o_;
     Np = Layout<G>::band_len(N);
-    nb = (M + G - 1) / G;
+    nb = (M + G * R - 1) / (G * R);
   }
   __device__ __host__ int pair_columns() const { return nb * Np; }
   __device__ __host__ int columns() const { return stream * nb * Np; }

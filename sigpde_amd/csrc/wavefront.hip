@@ -51,23 +51,24 @@ struct RefArith<double> {
 template <typename T, int G>
 using ProducerFor = typename std::conditional<G == 16, Global16Producer<T>, GlobalProducer<T, G>>::type;
 
-template <Arith A, int G, class Producer, class Out>
+template <Arith A, int G, int R, class Producer, class Out>
 __device__ __forceinline__ void solve_wave(Producer &P, const PairGeom<G> &g, double s, void *lds,
                                            Out &&out) {
   if constexpr (G == 16)
-    ring16_solve<A>(P, g, s, lds, out);
+    ring16_solve<A, R>(P, g, s, lds, out);
   else
     ring_solve<A, G>(P, g, s, lds, out);
 }
 
 // One wave per block; a wave solves Layout<G>::kPairs pairs (one per lane group of G lanes).
 // Increments stream from HBM through the wave's LDS ring.
-template <typename T, int G>
+// R: refined rows per lane of the 16-lane solver (1 for 64-lane groups).
+template <typename T, int G, int R>
 __global__ __launch_bounds__(kWave) void k_pde_pairwise(PairwiseArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using L = Layout<G>;
   const int lane = threadIdx.x & (kWave - 1);
-  PairGeom<G> g(a.m, a.n, a.order);
+  PairGeom<G> g(a.m, a.n, a.order, 1, R);
   ProducerFor<T, G> P;
   P.ld = a.n;
   P.scaled = a.sx || a.sy;
@@ -79,7 +80,7 @@ __global__ __launch_bounds__(kWave) void k_pde_pairwise(PairwiseArgs<T> a) {
     P.scale[q] = (P.scaled && ok) ? (a.sy ? a.sy[b] : (T)1) * (a.sx ? a.sx[b] : (T)1) : (T)1;
   }
   const int64_t b = (int64_t)blockIdx.x * L::kPairs + lane / G;
-  solve_wave<RefArith<T>::kPlain, G>(P, g, 1.0, smem_raw, [&](int, double r) {
+  solve_wave<RefArith<T>::kPlain, G, R>(P, g, 1.0, smem_raw, [&](int, double r) {
     if (b < a.B) a.out[b] = (T)r;
   });
 }
@@ -99,12 +100,12 @@ __device__ __forceinline__ void tile_pair(int64_t p, int64_t by, bool tri, int64
   }
 }
 
-template <typename T, int G>
+template <typename T, int G, int R>
 __global__ __launch_bounds__(kWave) void k_pde_gram(GramArgs<T> a, int64_t npairs, int tri) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using L = Layout<G>;
   const int lane = threadIdx.x & (kWave - 1);
-  PairGeom<G> g(a.m, a.n, a.order);
+  PairGeom<G> g(a.m, a.n, a.order, 1, R);
   ProducerFor<T, G> P;
   P.ld = a.n;
   P.scaled = a.sx || a.sy;
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(kWave) void k_pde_gram(GramArgs<T> a, int64_t npair
       xi = gi[w];
       yj = gj[w];
     }
-  solve_wave<RefArith<T>::kPlain, G>(P, g, 1.0, smem_raw, [&](int, double r) {
+  solve_wave<RefArith<T>::kPlain, G, R>(P, g, 1.0, smem_raw, [&](int, double r) {
     if (xi >= 0) {
       a.out[xi * a.ldo + yj] = (T)r;
       if (a.symmetric && xi != yj) a.out[yj * a.ldo + xi] = (T)r;
@@ -179,7 +180,7 @@ __device__ __forceinline__ double group_bcast(double v, int src) {
   return *reinterpret_cast<double *>(&r);
 }
 
-template <typename T, int G>
+template <typename T, int G, int R>
 __global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using L = Layout<G>;
@@ -187,8 +188,8 @@ __global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a) {
   const int q = lane / G;
   const int64_t b = (int64_t)blockIdx.x * L::kPairs + q;
   const bool live = b < a.B;
-  PairGeom<G> g(a.m, a.m, a.order);
-  const int rM = (g.M - 1) & (G - 1);
+  PairGeom<G> g(a.m, a.m, a.order, 1, R);
+  const int rM = G == 16 ? ((g.M - 1) % (16 * R)) / R : (g.M - 1) & (G - 1);
   ProducerFor<T, G> P;
   P.ld = a.m;
   P.scaled = false;
@@ -203,7 +204,7 @@ __global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a) {
   if (a.normalizer == 0) {
     // _norm_factors (sigpde/torch/kernel.py:181-184): K(x,x), NaN -> inf, target
     double r = 0.0;
-    solve_wave<RefArith<T>::kPlain, G>(P, g, 1.0, smem_raw, [&](int, double v) { r = v; });
+    solve_wave<RefArith<T>::kPlain, G, R>(P, g, 1.0, smem_raw, [&](int, double v) { r = v; });
     K1 = (T)group_bcast<G>(r, rM);
     if (isnan(K1)) K1 = (T)INFINITY;
     Tt = log_normalizer<T>(K1);
@@ -222,7 +223,7 @@ __global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a) {
   while (__any(active)) {
     double xt = av + t * (bv - av);
     double r = 0.0;
-    solve_wave<RefArith<T>::kScaled, G>(P, g, xt * xt, smem_raw, [&](int, double v) { r = v; });
+    solve_wave<RefArith<T>::kScaled, G, R>(P, g, xt * xt, smem_raw, [&](int, double v) { r = v; });
     double sol = group_bcast<G>(r, rM);
     if (!active) continue;
     ++it;
@@ -279,9 +280,15 @@ template <typename T>
 static hipError_t launch_pairwise(const PairwiseArgs<T> &a, hipStream_t st) {
   const int N = a.n << a.order;
   if (a.B >= kManyPairs) {
-    k_pde_pairwise<T, 16><<<dim3((unsigned)((a.B + 3) / 4)), dim3(kWave), lds_for<T, 16>(N), st>>>(a);
+    const dim3 grid((unsigned)((a.B + 3) / 4));
+    switch (r16_rows(a.order)) {
+      case 1: k_pde_pairwise<T, 16, 1><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
+      case 2: k_pde_pairwise<T, 16, 2><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
+      case 4: k_pde_pairwise<T, 16, 4><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
+      default: k_pde_pairwise<T, 16, 8><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
+    }
   } else {
-    k_pde_pairwise<T, 64><<<dim3((unsigned)a.B), dim3(kWave), lds_for<T, 64>(N), st>>>(a);
+    k_pde_pairwise<T, 64, 1><<<dim3((unsigned)a.B), dim3(kWave), lds_for<T, 64>(N), st>>>(a);
   }
   return hipGetLastError();
 }
@@ -292,9 +299,16 @@ static hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st) {
   const bool tri = a.symmetric && a.off_x == a.off_y && a.bx == a.by;
   const int64_t np = tri ? a.bx * (a.bx + 1) / 2 : a.bx * a.by;
   if (np >= kManyPairs) {
-    k_pde_gram<T, 16><<<dim3((unsigned)((np + 3) / 4)), dim3(kWave), lds_for<T, 16>(N), st>>>(a, np, (int)tri);
+    const dim3 grid((unsigned)((np + 3) / 4));
+    const size_t lds = lds_for<T, 16>(N);
+    switch (r16_rows(a.order)) {
+      case 1: k_pde_gram<T, 16, 1><<<grid, dim3(kWave), lds, st>>>(a, np, (int)tri); break;
+      case 2: k_pde_gram<T, 16, 2><<<grid, dim3(kWave), lds, st>>>(a, np, (int)tri); break;
+      case 4: k_pde_gram<T, 16, 4><<<grid, dim3(kWave), lds, st>>>(a, np, (int)tri); break;
+      default: k_pde_gram<T, 16, 8><<<grid, dim3(kWave), lds, st>>>(a, np, (int)tri); break;
+    }
   } else {
-    k_pde_gram<T, 64><<<dim3((unsigned)np), dim3(kWave), lds_for<T, 64>(N), st>>>(a, np, (int)tri);
+    k_pde_gram<T, 64, 1><<<dim3((unsigned)np), dim3(kWave), lds_for<T, 64>(N), st>>>(a, np, (int)tri);
   }
   return hipGetLastError();
 }
@@ -303,9 +317,15 @@ template <typename T>
 static hipError_t launch_robust(const RobustArgs<T> &a, hipStream_t st) {
   const int N = a.m << a.order;
   if (a.B >= kManyPairs) {
-    k_robust_norm<T, 16><<<dim3((unsigned)((a.B + 3) / 4)), dim3(kWave), lds_for<T, 16>(N), st>>>(a);
+    const dim3 grid((unsigned)((a.B + 3) / 4));
+    switch (r16_rows(a.order)) {
+      case 1: k_robust_norm<T, 16, 1><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
+      case 2: k_robust_norm<T, 16, 2><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
+      case 4: k_robust_norm<T, 16, 4><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
+      default: k_robust_norm<T, 16, 8><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
+    }
   } else {
-    k_robust_norm<T, 64><<<dim3((unsigned)a.B), dim3(kWave), lds_for<T, 64>(N), st>>>(a);
+    k_robust_norm<T, 64, 1><<<dim3((unsigned)a.B), dim3(kWave), lds_for<T, 64>(N), st>>>(a);
   }
   return hipGetLastError();
 }
