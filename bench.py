@@ -14,10 +14,17 @@ Besides the JSON contract fields the line carries:
   rel_err_fp64   ||K - K64||_F / ||K64||_F on a sampled block, K64 = the float64 oracle on the same
                  float32 inputs upcast (computed in the CPU-baseline leg, which runs the oracle);
                  the same figure is given for the two comparison paths
+  rel_err_fp64_full  the same norm over all entries, K64 = precision="ref" on the float64 inputs
+                 (the reference scheme in float64 on the GPU)
   latency_ms     p50 / p99 of the per-step times
   roofline       dominant kernel: achieved vs peak, measured with HIP events on its stream
   cpu_baseline   the oracle (C restatement, float32 reference semantics) timed on this host's
                  cores on a sampled block, in pairs/sec
+  workloads      the smaller workloads of the API, each with its float64-oracle error: float64
+                 pairwise (8 pairs, length 32, dim 4, order 1: per-call latency p50 / p99), float32
+                 Gram 128 x 128 (length 100, dim 8, order 2: every precision), float64
+                 RobustSigPDE (256 paths, length 64, dim 16, order 1: normalization and Gram
+                 times, root-find iteration histogram)
   serving        the continuous-batching request loop (sigpde_amd.serving) on a bounded sample of
                  the request mix (70% SigPDE / 30% RobustSigPDE, pairwise or Gram, batch 1-256,
                  length 16-512, dim 4/16/64, dyadic order 0-3, float32 Brownian paths, 64 GB HBM
@@ -71,6 +78,10 @@ def parse():
     p.add_argument("--sample", type=int, default=32, help="side of the sampled accuracy block")
     p.add_argument("--no-compare", action="store_true",
                    help="time only --mode (skip the plain-FP32 and reference-scheme legs)")
+    p.add_argument("--no-full-check", action="store_true",
+                   help="skip the all-entries comparison with the float64 GPU path")
+    p.add_argument("--no-workloads", action="store_true",
+                   help="skip the smaller API workloads (pairwise, Gram, robust) beside the line")
     p.add_argument("--serving", type=int, default=80, metavar="N",
                    help="requests of the serving leg (0: skip it)")
     p.add_argument("--serving-rate", type=float, default=5.0,
@@ -243,6 +254,114 @@ def cpu_leg(args, X, Y, K, others):
     }
 
 
+def _rel(K, K64):
+    K = np.asarray(K, dtype=np.float64)
+    return float(np.linalg.norm(K - K64) / np.linalg.norm(K64))
+
+
+def _wall_ms(fn, reps):
+    """Per-call wall times (ms), each call followed by a device synchronise (the reference returns
+    synchronised)."""
+    out = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        out.append((time.perf_counter() - t0) * 1e3)
+    return out
+
+
+def _event_ms(fn, reps):
+    """Mean device time (ms) of reps back-to-back calls, CUDA events around the whole run."""
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        out = fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return out, e0.elapsed_time(e1) / reps
+
+
+def workloads_leg(args, dev):
+    """The three smaller workloads of the reference's API, each through the public API and beside
+    its float64-oracle error (the oracle runs on the same inputs, upcast for float32 ones):
+      pairwise  SigPDE(LinearKernel(), 1).pairwise(x, y), float64, 8 pairs, length 32, dim 4
+                (latency-bound: per-call wall time p50 / p99, synchronised on return)
+      gram      SigPDE(LinearKernel(), 2).gram(X, Y), float32, 128 x 128, length 100, dim 8
+                (reference scheme and the fused paths; full 128 x 128 float64 oracle)
+      robust    RobustSigPDE(LinearKernel(), 1): .normalization(X) and .gram(X), float64, 256 paths,
+                length 64, dim 16, increments N(0, 9/64), tol 1e-10 (root-find iteration histogram)
+    """
+    import oracle
+
+    oracle.set_num_threads(min(os.cpu_count() or 1, 16))
+    lin = sig.kernels.LinearKernel()
+    out = {}
+
+    # pairwise, float64, order 1
+    x = brownian_paths(8, 32, 4, 1.0 / 32, 0, torch.float64, dev)
+    y = brownian_paths(8, 32, 4, 1.0 / 32, 1, torch.float64, dev)
+    k1 = sig.SigPDE(lin, 1)
+    for _ in range(5):
+        k1.pairwise(x, y)
+    torch.cuda.synchronize()
+    lat = _wall_ms(lambda: k1.pairwise(x, y), 200)
+    K = k1.pairwise(x, y).cpu().numpy()
+    K64 = oracle.sig_pairwise(x.cpu().numpy(), y.cpu().numpy(), 1)
+    out["pairwise_8_L32_d4_o1_f64"] = {
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "calls": len(lat), "rel_err_fp64": _rel(K, K64),
+        "max_abs_err_fp64": float(np.abs(K - K64).max())}
+
+    # Gram, float32, order 2
+    X = brownian_paths(128, 100, 8, 1.0 / 100, 2, torch.float32, dev)
+    Y = brownian_paths(128, 100, 8, 1.0 / 100, 3, torch.float32, dev)
+    K64 = oracle.linear_gram(X.double().cpu().numpy(), Y.double().cpu().numpy(), 2)
+    K32ref = oracle.linear_gram(X.cpu().numpy(), Y.cpu().numpy(), 2)  # float32 reference semantics
+    g = {}
+    for mode in ("ref", "fp32", "fp16x2", "bf16x3"):
+        k2 = sig.SigPDE(lin, 2, precision=mode)
+        k2.gram(X, Y)
+        torch.cuda.synchronize()
+        Kg, ms = _event_ms(lambda: k2.gram(X, Y), 5)
+        Kg = Kg.double().cpu().numpy()
+        g[mode] = {"ms": ms, "pairs_per_s": 128 * 128 / (ms / 1e3), "rel_err_fp64": _rel(Kg, K64)}
+        if mode == "ref":
+            g[mode]["rel_err_vs_fp32_reference_semantics"] = _rel(Kg, K32ref.astype(np.float64))
+    out["gram_128x128_L100_d8_o2_f32"] = g
+
+    # robust, float64, order 1
+    Xr = brownian_paths(256, 64, 16, 9.0 / 64, 4, torch.float64, dev)
+    k3 = sig.RobustSigPDE(lin, 1)
+    k3.normalization(Xr, tol=1e-10)
+    torch.cuda.synchronize()
+    lam, ms_n = _event_ms(lambda: k3.normalization(Xr, tol=1e-10), 3)
+    Kr, ms_g = _event_ms(lambda: k3.gram(Xr, tol=1e-10), 3)
+    iters = torch.zeros(256, dtype=torch.int32, device=dev)
+    k3._normalization(Xr, k3.normalizer, 1e-10, 100, 1000, iters=iters)
+    it = iters.cpu().numpy()
+    Xn = Xr.cpu().numpy()
+    lam64 = oracle.robust_normalization(Xn, 1, tol=1e-10, maxit=100)
+    Kr64 = oracle.linear_gram(Xn, Xn, 1, sx=lam64, sy=lam64)
+    # the final scaled Gram solves on the fused fp16x2 path (root-find still float64)
+    k3m = sig.RobustSigPDE(lin, 1, precision="fp16x2")
+    k3m.gram(Xr, tol=1e-10)
+    torch.cuda.synchronize()
+    Krm, ms_gm = _event_ms(lambda: k3m.gram(Xr, tol=1e-10), 3)
+    lam = lam.cpu().numpy()
+    hist = np.bincount(it, minlength=int(it.max()) + 1)
+    out["robust_256_L64_d16_o1_f64"] = {
+        "normalization_ms": ms_n, "gram_ms": ms_g,
+        "gram_pairs_per_s": 256 * 256 / (ms_g / 1e3),
+        "lambda_max_abs_err_fp64": float(np.abs(lam - lam64).max()),
+        "gram_rel_err_fp64": _rel(Kr.cpu().numpy(), Kr64),
+        "gram_fp16x2_ms": ms_gm, "gram_fp16x2_rel_err_fp64": _rel(Krm.cpu().numpy(), Kr64),
+        "active_paths": int((lam64 < 1).sum()),
+        "iterations_histogram": {str(i): int(c) for i, c in enumerate(hist) if c}}
+    return out
+
+
 def serving_leg(args, dev):
     """Serves a bounded sample of the request mix with Poisson arrivals; every SigPDE and robust
     final solve runs in precision fp16x2 (the robust root-find in the reference's arithmetic on the
@@ -343,6 +462,29 @@ def main():
         for name, rel in rel_others.items():
             line[name]["rel_err_fp64"] = rel
         line["cpu_baseline"] = cpu
+    if not args.no_full_check:
+        # every entry of the Gram against the float64 GPU path (the reference scheme in float64 on
+        # the same float32 inputs upcast; pinned to the CPU oracle by the GPU parity tests)
+        t0 = time.perf_counter()
+        K64 = sig.SigPDE(sig.kernels.LinearKernel(), args.order).gram(
+            X.double(), Y.double(), max_batch=args.max_batch)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        n64 = torch.linalg.norm(K64)
+
+        def rel_full(Kc):
+            return float(torch.linalg.norm(Kc.double() - K64) / n64)
+
+        line["rel_err_fp64_full"] = rel_full(K)
+        line["rel_err_fp64_full_reference"] = (
+            f"all {pairs} entries vs precision='ref' on float64 inputs (GPU, {t64:.1f} s)")
+        for name, Ko in (("plain_fp32", K32), ("reference_scheme_fp32", Kref),
+                         ("mixed_bf16x3", Kb)):
+            if Ko is not K:
+                line[name]["rel_err_fp64_full"] = rel_full(Ko)
+        del K64
+    if not args.no_workloads:
+        line["workloads"] = workloads_leg(args, dev)
     if args.serving > 0:
         line["serving"] = serving_leg(args, dev)
     print(json.dumps(line), flush=True)

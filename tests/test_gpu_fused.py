@@ -127,3 +127,21 @@ def test_fused_fp32_matches_fp32_reference_scheme_on_exact_increments():
 def test_fused_rejects_nonlinear_kernel(prec):
     with pytest.raises(ValueError):
         sig.SigPDE(sig.kernels.RBFKernel(), 0, precision=prec)
+
+
+@pytest.mark.parametrize("prec", ["fp16x2", "bf16x3"])
+def test_fused_full_1024x1024_gram_vs_gpu_float64(prec):
+    """Every entry of the 1024 x 1024 Gram (length 256, dim 64, order 0) against the float64 GPU
+    path (precision="ref" on the float64 inputs: the reference scheme in float64, pinned to the CPU
+    oracle at 1e-10 by test_gpu_wavefront.py / test_gpu_api.py); the oracle itself is checked on a
+    16 x 16 corner here as well."""
+    from sigpde_amd.synthetic import brownian_paths
+    X = brownian_paths(1024, 256, 64, 1 / 256, 5, torch.float32, DEV)
+    Y = brownian_paths(1024, 256, 64, 1 / 256, 6, torch.float32, DEV)
+    lin = sig.kernels.LinearKernel()
+    K = sig.SigPDE(lin, 0, precision=prec).gram(X, Y)
+    K64 = sig.SigPDE(lin, 0).gram(X.double(), Y.double(), max_batch=128 * 128)
+    ref_corner = O.linear_gram(X[:16].double().cpu().numpy(), Y[:16].double().cpu().numpy(), 0)
+    assert rel_fro(K64[:16, :16].cpu().numpy(), ref_corner) < 1e-12
+    rel = float(torch.linalg.norm(K.double() - K64) / torch.linalg.norm(K64))
+    assert rel < BARS[prec], rel
