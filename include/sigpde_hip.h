@@ -11,7 +11,11 @@
  *     message for the calling thread.
  *   - Refined lengths follow the reference: Lref = ((L - 1) << order) + 1
  *     (sigpde/utils.py:6-7); with m = Lx - 1 and n = Ly - 1 increments the refined grid of one
- *     pair has (m << order) x (n << order) cells. Supported: (m << order) <= 4096 rows.
+ *     pair has (m << order) x (n << order) cells. Supported: (m << order) <= 2^20 refined
+ *     rows; refined columns (n << order) up to the boundary lines one wave keeps in LDS (160 KB
+ *     on gfx950): 17695 (SIGPDE_F32) / 14927 (SIGPDE_F64) in the entry points that take
+ *     increments, 9679 in the fused linear ones. Longer inputs return SIGPDE_EINVAL naming the
+ *     limit.
  *
  * The reference has no C interface: its "FFI" is the Python/Numba launch layer
  * sigpde/PDESolvers.py, whose kernel launches these entry points replace one for one.

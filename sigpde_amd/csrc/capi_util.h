@@ -9,7 +9,12 @@
 
 namespace sigpde {
 
-constexpr int kMaxRefinedRows = 4096;
+// Refined rows (m << order) of one pair: bounded only so that the kernels' virtual-column
+// counts (bands x band length x pairs per stream) stay within int.
+constexpr int64_t kMaxRefinedRows = 1 << 20;
+// LDS of one workgroup on gfx950 (160 KB). It bounds the refined columns (n << order) of a pair:
+// the wave keeps its band-boundary lines, one value per refined column, in LDS.
+constexpr size_t kMaxLdsBytes = 163840;
 
 char *error_buffer();  // thread-local, defined in capi.hip
 

@@ -293,6 +293,11 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
     const int64_t nq = (Bx + 3) / 4;
     int ys = 16;
     while (ys > 1 && nq * ((By + ys - 1) / ys) < 4096) ys >>= 1;
+    // a wave's virtual-column count (ys pairs x bands x band length) must stay within int
+    {
+      const PairGeom<16> g1(a.m, a.n, order, 1, r16_rows(order));
+      while (ys > 1 && (int64_t)ys * g1.pair_columns() > (int64_t)1 << 30) ys >>= 1;
+    }
     a.ys = ys;
     const int64_t nyr = (By + ys - 1) / ys;
     // super-tiles of about one XCD's resident waves
@@ -351,6 +356,16 @@ static hipError_t dispatch_prec(int precision, bool pairwise, const void *X, con
 
 constexpr int64_t kMaxDim = 4 * kKBlock;
 
+// Longest refined column count whose four float boundary lines and ring fit one wave's LDS.
+static int64_t fused_max_columns() {
+  int64_t lo = 1, hi = 1 << 20;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) / 2;
+    (r16_lds_bytes<Arith::Mixed>((int)mid) <= kMaxLdsBytes ? lo : hi) = mid;
+  }
+  return lo;
+}
+
 static int check_fused(int dtype, int precision, const void *X, const void *Y, int64_t Bx,
                        int64_t By, int64_t Lx, int64_t Ly, int64_t d, int order, void *out,
                        void *ws) {
@@ -364,8 +379,15 @@ static int check_fused(int dtype, int precision, const void *X, const void *Y, i
   if (Lx < 2 || Ly < 2) return set_error(SIGPDE_EINVAL, "linear: paths need length >= 2");
   if (d < 1 || d > kMaxDim) return set_error(SIGPDE_EINVAL, "linear: dim must be in [1, %d]", (int)kMaxDim);
   if (order < 0 || order > 12) return set_error(SIGPDE_EINVAL, "linear: dyadic order must be in [0, 12]");
-  if (((Lx - 1) << order) > kMaxRefinedRows || ((Ly - 1) << order) > kMaxRefinedRows)
-    return set_error(SIGPDE_EINVAL, "linear: refined grid side (length-1) << order exceeds %d", kMaxRefinedRows);
+  if (((Lx - 1) << order) > kMaxRefinedRows)
+    return set_error(SIGPDE_EINVAL, "linear: refined rows (Lx-1) << order = %lld exceed %lld",
+                     (long long)((Lx - 1) << order), (long long)kMaxRefinedRows);
+  const int64_t maxc = fused_max_columns();
+  if (((Ly - 1) << order) > maxc)
+    return set_error(SIGPDE_EINVAL,
+                     "linear: refined columns (Ly-1) << order = %lld exceed %lld, the longest "
+                     "boundary lines one wave's LDS holds",
+                     (long long)((Ly - 1) << order), (long long)maxc);
   return SIGPDE_OK;
 }
 

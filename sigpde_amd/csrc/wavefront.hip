@@ -276,10 +276,26 @@ static size_t lds_for(int N) {
   return wave_lds_bytes<RefArith<T>::kPlain, G>(N);
 }
 
+// Lane groups of 16 keep four boundary lines per wave, groups of 64 one: past the refined length
+// whose four lines fill the LDS, every batch runs on groups of 64.
+template <typename T>
+static bool g16_fits(int N) { return lds_for<T, 16>(N) <= kMaxLdsBytes; }
+
+// Longest refined column count (n << order) that groups of 64 hold in LDS.
+template <typename T>
+static int64_t max_columns() {
+  int64_t lo = 1, hi = 1 << 20;  // lds_for<T, 64>(lo) fits, lds_for<T, 64>(hi) does not
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) / 2;
+    (lds_for<T, 64>((int)mid) <= kMaxLdsBytes ? lo : hi) = mid;
+  }
+  return lo;
+}
+
 template <typename T>
 static hipError_t launch_pairwise(const PairwiseArgs<T> &a, hipStream_t st) {
   const int N = a.n << a.order;
-  if (a.B >= kManyPairs) {
+  if (a.B >= kManyPairs && g16_fits<T>(N)) {
     const dim3 grid((unsigned)((a.B + 3) / 4));
     switch (r16_rows(a.order)) {
       case 1: k_pde_pairwise<T, 16, 1><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
@@ -298,7 +314,7 @@ static hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st) {
   const int N = a.n << a.order;
   const bool tri = a.symmetric && a.off_x == a.off_y && a.bx == a.by;
   const int64_t np = tri ? a.bx * (a.bx + 1) / 2 : a.bx * a.by;
-  if (np >= kManyPairs) {
+  if (np >= kManyPairs && g16_fits<T>(N)) {
     const dim3 grid((unsigned)((np + 3) / 4));
     const size_t lds = lds_for<T, 16>(N);
     switch (r16_rows(a.order)) {
@@ -316,7 +332,7 @@ static hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st) {
 template <typename T>
 static hipError_t launch_robust(const RobustArgs<T> &a, hipStream_t st) {
   const int N = a.m << a.order;
-  if (a.B >= kManyPairs) {
+  if (a.B >= kManyPairs && g16_fits<T>(N)) {
     const dim3 grid((unsigned)((a.B + 3) / 4));
     switch (r16_rows(a.order)) {
       case 1: k_robust_norm<T, 16, 1><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
@@ -330,11 +346,19 @@ static hipError_t launch_robust(const RobustArgs<T> &a, hipStream_t st) {
   return hipGetLastError();
 }
 
-static int check_grid(int64_t m, int64_t n, int order) {
+// m: increments along the rows (bands), n: along the columns (boundary lines).
+static int check_grid(int dtype, int64_t m, int64_t n, int order) {
   if (m < 1 || n < 1) return set_error(SIGPDE_EINVAL, "paths need length >= 2 (m, n >= 1 increments)");
   if (order < 0 || order > 12) return set_error(SIGPDE_EINVAL, "dyadic order must be in [0, 12]");
-  if ((m << order) > kMaxRefinedRows || (n << order) > kMaxRefinedRows)
-    return set_error(SIGPDE_EINVAL, "refined grid side (length-1) << order exceeds %d", kMaxRefinedRows);
+  if ((m << order) > kMaxRefinedRows)
+    return set_error(SIGPDE_EINVAL, "refined rows (length-1) << order = %lld exceed %lld",
+                     (long long)(m << order), (long long)kMaxRefinedRows);
+  const int64_t maxc = dtype == SIGPDE_F32 ? max_columns<float>() : max_columns<double>();
+  if ((n << order) > maxc)
+    return set_error(SIGPDE_EINVAL,
+                     "refined columns (length-1) << order = %lld exceed %lld, the longest "
+                     "boundary line one wave's LDS holds for this dtype",
+                     (long long)(n << order), (long long)maxc);
   return SIGPDE_OK;
 }
 
@@ -348,7 +372,7 @@ extern "C" int sigpde_pde_pairwise(int dtype, const void *inc, int64_t B, int64_
   clear_error();
   if (B == 0) return SIGPDE_OK;
   if (B < 0 || !inc || !out) return set_error(SIGPDE_EINVAL, "pde_pairwise: bad batch or null pointer");
-  if (int rc = check_grid(m, n, order)) return rc;
+  if (int rc = check_grid(dtype, m, n, order)) return rc;
   hipStream_t st = (hipStream_t)stream;
   hipError_t e;
   if (dtype == SIGPDE_F32) {
@@ -372,7 +396,7 @@ extern "C" int sigpde_pde_gram(int dtype, const void *inc, int64_t bx, int64_t b
   clear_error();
   if (bx == 0 || by == 0) return SIGPDE_OK;
   if (bx < 0 || by < 0 || !inc || !out) return set_error(SIGPDE_EINVAL, "pde_gram: bad tile or null pointer");
-  if (int rc = check_grid(m, n, order)) return rc;
+  if (int rc = check_grid(dtype, m, n, order)) return rc;
   if (symmetric && m != n) return set_error(SIGPDE_EINVAL, "pde_gram: symmetric Gram needs m == n");
   hipStream_t st = (hipStream_t)stream;
   hipError_t e;
@@ -404,7 +428,7 @@ extern "C" int sigpde_robust_norm(int dtype, const void *inc, int64_t B, int64_t
   if (normalizer == SIGPDE_NORM_GIVEN && !target)
     return set_error(SIGPDE_EINVAL, "robust_norm: SIGPDE_NORM_GIVEN needs target");
   if (maxit < 0) return set_error(SIGPDE_EINVAL, "robust_norm: maxit must be >= 0");
-  if (int rc = check_grid(m, m, order)) return rc;
+  if (int rc = check_grid(dtype, m, m, order)) return rc;
   hipStream_t st = (hipStream_t)stream;
   hipError_t e;
   if (dtype == SIGPDE_F32) {

@@ -418,7 +418,7 @@ def synthetic_requests(n: int, seed: int = 7, rate: float = math.inf, device="cu
     """Request mix: SigPDE vs RobustSigPDE (robust_frac), pairwise vs Gram (gram_frac), batch
     U{1..max_batch_size}, length U{lengths}, dim from dims, order from orders, increments
     N(0, 1/length); Poisson arrivals at `rate` per second (inf: all at t = 0). Orders are capped
-    so that the refined grid side stays within the library's 4096."""
+    so that the refined grid side stays within 4096 (the largest of the request mix)."""
     rng = np.random.default_rng(seed)
     out = []
     t = 0.0

@@ -124,8 +124,8 @@ def test_empty_batch_is_noop():
 def test_bad_arguments_raise():
     inc = torch.zeros((2, 5, 5), dtype=torch.float32, device=DEV)
     out = torch.zeros(2, dtype=torch.float32, device=DEV)
-    with pytest.raises(_native.NativeError, match="exceeds"):
-        _native.pde_pairwise(inc, 10, out)  # 5 << 10 rows > 4096
+    with pytest.raises(_native.NativeError, match="refined columns"):
+        _native.pde_pairwise(inc, 12, out)  # 5 << 12 = 20480 refined columns: past the LDS lines
     with pytest.raises(TypeError):
         _native.pde_pairwise(inc.half(), 0, out)
 
