@@ -12,10 +12,12 @@
  *   - Refined lengths follow the reference: Lref = ((L - 1) << order) + 1
  *     (sigpde/utils.py:6-7); with m = Lx - 1 and n = Ly - 1 increments the refined grid of one
  *     pair has (m << order) x (n << order) cells. Supported: (m << order) <= 2^20 refined
- *     rows; refined columns (n << order) up to the boundary lines one wave keeps in LDS (160 KB
- *     on gfx950): 17695 (SIGPDE_F32) / 14927 (SIGPDE_F64) in the entry points that take
- *     increments, 9679 in the fused linear ones. Longer inputs return SIGPDE_EINVAL naming the
- *     limit.
+ *     rows and (n << order) <= 2^22 refined columns, with (rows / 16 + 1) x (columns + 48)
+ *     <= 2^30, in the entry points that take increments (past 17695 SIGPDE_F32 / 14927
+ *     SIGPDE_F64 refined columns the band-boundary lines go to global memory, allocated
+ *     stream-ordered by the call: hipMallocAsync / hipFreeAsync on `stream`); the fused linear
+ *     entry points take up to 9679 refined columns (their lines stay in LDS). Larger inputs
+ *     return SIGPDE_EINVAL naming the limit.
  *
  * The reference has no C interface: its "FFI" is the Python/Numba launch layer
  * sigpde/PDESolvers.py, whose kernel launches these entry points replace one for one.

@@ -9,11 +9,13 @@
 
 namespace sigpde {
 
-// Refined rows (m << order) of one pair: bounded only so that the kernels' virtual-column
-// counts (bands x band length x pairs per stream) stay within int.
+// Refined rows (m << order) and columns (n << order) of one pair. Rows cost nothing on chip;
+// columns are held by the band-boundary lines (LDS, or global memory past what LDS holds). Both
+// are bounded so that a wave's virtual-column count (bands x band length) stays within int.
 constexpr int64_t kMaxRefinedRows = 1 << 20;
-// LDS of one workgroup on gfx950 (160 KB). It bounds the refined columns (n << order) of a pair:
-// the wave keeps its band-boundary lines, one value per refined column, in LDS.
+constexpr int64_t kMaxRefinedCols = 1 << 22;
+constexpr int64_t kMaxVirtualColumns = (int64_t)1 << 30;
+// LDS of one workgroup on gfx950 (160 KB).
 constexpr size_t kMaxLdsBytes = 163840;
 
 char *error_buffer();  // thread-local, defined in capi.hip

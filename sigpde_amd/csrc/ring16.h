@@ -71,6 +71,19 @@ __host__ __device__ inline size_t r16_lds_bytes(int N) {
          (size_t)kR16Slots * kR16Stride * sizeof(RT);
 }
 
+// LDS of one wave whose boundary lines live in global memory (long rows of columns): the ring.
+template <Arith A>
+__host__ __device__ inline size_t r16_ring_bytes() {
+  return (size_t)kR16Slots * kR16Stride * sizeof(typename ArithTraits<A>::RT);
+}
+
+// Global memory for the four boundary lines of one wave (same layout as in LDS).
+template <Arith A>
+__host__ __device__ inline size_t r16_line_bytes(int N) {
+  using BT = typename R16Bnd<A>::T;
+  return (size_t)4 * r16_line_stride<A>(Layout<16>::band_len(N)) * sizeof(BT);
+}
+
 // Chunk writes of a producer. The producing lane l = (c, h), c = l & 15, h = l >> 4, holds element
 // e (0..3) of pair g: ring row 16g + 4h + e, chunk column c. Chunk k goes to slot
 // (16 (k & 1) + c + 4h + e) & 31. Consecutive e are consecutive slots, so one base offset per
@@ -293,17 +306,29 @@ __device__ __forceinline__ void r16_head(const char *rb, const R16Period<typenam
 //      lane_scale (Arith::Mixed: the scale of the pair whose chunk was multiplied last, read at
 //      the start of the period that holds that pair's first reset)
 //   s: per-lane double scale of the Scaled* modes (ignored otherwise)
-//   lds: r16_lds_bytes<A>(N) bytes of this wave's LDS
-template <Arith A, int R, class Producer, class Out>
-__device__ void ring16_solve(Producer &P, const PairGeom<16> &g, double s, void *lds, Out &&out) {
+//   lds: r16_lds_bytes<A>(N) bytes of this wave's LDS (GL: r16_ring_bytes<A>())
+//   GL, gline: the four boundary lines in global memory at gline (r16_line_bytes<A>(N) bytes,
+//      this wave's own) instead of LDS, for rows of columns longer than LDS holds. A line value is
+//      read Np - 15 >= 17 steps after the same wave stored it; a wave's memory instructions reach
+//      an address in program order, so the read sees the store.
+template <Arith A, int R, bool GL = false, class Producer, class Out>
+__device__ void ring16_solve(Producer &P, const PairGeom<16> &g, double s, void *lds, Out &&out,
+                             typename R16Bnd<A>::T *gline = nullptr) {
   using RT = typename ArithTraits<A>::RT;
   using BT = typename R16Bnd<A>::T;
   const int lane = threadIdx.x & 63;
   const int rho = lane & 15;
   const int Np = g.Np;
   const int bs = r16_line_stride<A>(Np);
-  BT *bnd = reinterpret_cast<BT *>(lds);
-  RT *ring = reinterpret_cast<RT *>(bnd + 4 * bs);
+  BT *bnd;
+  RT *ring;
+  if constexpr (GL) {
+    bnd = gline;
+    ring = reinterpret_cast<RT *>(lds);
+  } else {
+    bnd = reinterpret_cast<BT *>(lds);
+    ring = reinterpret_cast<RT *>(bnd + 4 * bs);
+  }
   BT *line = bnd + (lane >> 4) * bs;
   const char *rb = reinterpret_cast<const char *>(ring + lane);
 

@@ -51,36 +51,52 @@ struct RefArith<double> {
 template <typename T, int G>
 using ProducerFor = typename std::conditional<G == 16, Global16Producer<T>, GlobalProducer<T, G>>::type;
 
-template <Arith A, int G, int R, class Producer, class Out>
+// GL: boundary lines in global memory (16-lane groups only), `lines` = this wave's block of
+// r16_line_bytes<A>(N) bytes.
+template <Arith A, int G, int R, bool GL, class Producer, class Out>
 __device__ __forceinline__ void solve_wave(Producer &P, const PairGeom<G> &g, double s, void *lds,
-                                           Out &&out) {
+                                           void *lines, Out &&out) {
+  static_assert(!GL || G == 16, "global boundary lines need 16-lane groups");
   if constexpr (G == 16)
-    ring16_solve<A, R>(P, g, s, lds, out);
+    ring16_solve<A, R, GL>(P, g, s, lds, out, reinterpret_cast<typename R16Bnd<A>::T *>(lines));
   else
     ring_solve<A, G>(P, g, s, lds, out);
+}
+
+// This wave's block of global boundary lines (GL launches: `gl` holds one block per wave of the
+// launch; other launches pass null).
+template <Arith A, bool GL>
+__device__ __forceinline__ void *wave_lines(void *gl, int N) {
+  if constexpr (GL)
+    return static_cast<char *>(gl) + (size_t)blockIdx.x * r16_line_bytes<A>(N);
+  else
+    return nullptr;
 }
 
 // One wave per block; a wave solves Layout<G>::kPairs pairs (one per lane group of G lanes).
 // Increments stream from HBM through the wave's LDS ring.
 // R: refined rows per lane of the 16-lane solver (1 for 64-lane groups).
-template <typename T, int G, int R>
-__global__ __launch_bounds__(kWave) void k_pde_pairwise(PairwiseArgs<T> a) {
+// w0: index of the launch's first wave (launches of global-line chunks), gl: global lines (GL).
+template <typename T, int G, int R, bool GL>
+__global__ __launch_bounds__(kWave) void k_pde_pairwise(PairwiseArgs<T> a, int64_t w0, void *gl) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using L = Layout<G>;
   const int lane = threadIdx.x & (kWave - 1);
+  const int64_t wv = w0 + blockIdx.x;
   PairGeom<G> g(a.m, a.n, a.order, 1, R);
   ProducerFor<T, G> P;
   P.ld = a.n;
   P.scaled = a.sx || a.sy;
 #pragma unroll
   for (int q = 0; q < L::kPairs; ++q) {
-    const int64_t b = (int64_t)blockIdx.x * L::kPairs + q;
+    const int64_t b = wv * L::kPairs + q;
     const bool ok = b < a.B;
     P.inc[q] = ok ? a.inc + b * a.stride_b : nullptr;
     P.scale[q] = (P.scaled && ok) ? (a.sy ? a.sy[b] : (T)1) * (a.sx ? a.sx[b] : (T)1) : (T)1;
   }
-  const int64_t b = (int64_t)blockIdx.x * L::kPairs + lane / G;
-  solve_wave<RefArith<T>::kPlain, G, R>(P, g, 1.0, smem_raw, [&](int, double r) {
+  const int64_t b = wv * L::kPairs + lane / G;
+  constexpr Arith A = RefArith<T>::kPlain;
+  solve_wave<A, G, R, GL>(P, g, 1.0, smem_raw, wave_lines<A, GL>(gl, g.N), [&](int, double r) {
     if (b < a.B) a.out[b] = (T)r;
   });
 }
@@ -100,11 +116,13 @@ __device__ __forceinline__ void tile_pair(int64_t p, int64_t by, bool tri, int64
   }
 }
 
-template <typename T, int G, int R>
-__global__ __launch_bounds__(kWave) void k_pde_gram(GramArgs<T> a, int64_t npairs, int tri) {
+template <typename T, int G, int R, bool GL>
+__global__ __launch_bounds__(kWave) void k_pde_gram(GramArgs<T> a, int64_t npairs, int tri,
+                                                    int64_t w0, void *gl) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using L = Layout<G>;
   const int lane = threadIdx.x & (kWave - 1);
+  const int64_t wv = w0 + blockIdx.x;
   PairGeom<G> g(a.m, a.n, a.order, 1, R);
   ProducerFor<T, G> P;
   P.ld = a.n;
@@ -112,7 +130,7 @@ __global__ __launch_bounds__(kWave) void k_pde_gram(GramArgs<T> a, int64_t npair
   int64_t gi[L::kPairs], gj[L::kPairs];
 #pragma unroll
   for (int q = 0; q < L::kPairs; ++q) {
-    const int64_t p = (int64_t)blockIdx.x * L::kPairs + q;
+    const int64_t p = wv * L::kPairs + q;
     int64_t i = 0, j = 0;
     bool ok = p < npairs;
     if (ok) tile_pair(p, a.by, tri, i, j);
@@ -131,7 +149,8 @@ __global__ __launch_bounds__(kWave) void k_pde_gram(GramArgs<T> a, int64_t npair
       xi = gi[w];
       yj = gj[w];
     }
-  solve_wave<RefArith<T>::kPlain, G, R>(P, g, 1.0, smem_raw, [&](int, double r) {
+  constexpr Arith A = RefArith<T>::kPlain;
+  solve_wave<A, G, R, GL>(P, g, 1.0, smem_raw, wave_lines<A, GL>(gl, g.N), [&](int, double r) {
     if (xi >= 0) {
       a.out[xi * a.ldo + yj] = (T)r;
       if (a.symmetric && xi != yj) a.out[yj * a.ldo + xi] = (T)r;
@@ -180,13 +199,14 @@ __device__ __forceinline__ double group_bcast(double v, int src) {
   return *reinterpret_cast<double *>(&r);
 }
 
-template <typename T, int G, int R>
-__global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a) {
+template <typename T, int G, int R, bool GL>
+__global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a, int64_t w0, void *gl) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using L = Layout<G>;
   const int lane = threadIdx.x & (kWave - 1);
   const int q = lane / G;
-  const int64_t b = (int64_t)blockIdx.x * L::kPairs + q;
+  const int64_t wv = w0 + blockIdx.x;
+  const int64_t b = wv * L::kPairs + q;
   const bool live = b < a.B;
   PairGeom<G> g(a.m, a.m, a.order, 1, R);
   const int rM = G == 16 ? ((g.M - 1) % (16 * R)) / R : (g.M - 1) & (G - 1);
@@ -195,7 +215,7 @@ __global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a) {
   P.scaled = false;
 #pragma unroll
   for (int w = 0; w < L::kPairs; ++w) {
-    const int64_t bw = (int64_t)blockIdx.x * L::kPairs + w;
+    const int64_t bw = wv * L::kPairs + w;
     P.inc[w] = bw < a.B ? a.inc + bw * a.stride_b : nullptr;
     P.scale[w] = (T)1;
   }
@@ -204,7 +224,8 @@ __global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a) {
   if (a.normalizer == 0) {
     // _norm_factors (sigpde/torch/kernel.py:181-184): K(x,x), NaN -> inf, target
     double r = 0.0;
-    solve_wave<RefArith<T>::kPlain, G, R>(P, g, 1.0, smem_raw, [&](int, double v) { r = v; });
+    solve_wave<RefArith<T>::kPlain, G, R, GL>(
+        P, g, 1.0, smem_raw, wave_lines<RefArith<T>::kPlain, GL>(gl, g.N), [&](int, double v) { r = v; });
     K1 = (T)group_bcast<G>(r, rM);
     if (isnan(K1)) K1 = (T)INFINITY;
     Tt = log_normalizer<T>(K1);
@@ -223,7 +244,8 @@ __global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a) {
   while (__any(active)) {
     double xt = av + t * (bv - av);
     double r = 0.0;
-    solve_wave<RefArith<T>::kScaled, G, R>(P, g, xt * xt, smem_raw, [&](int, double v) { r = v; });
+    solve_wave<RefArith<T>::kScaled, G, R, GL>(
+        P, g, xt * xt, smem_raw, wave_lines<RefArith<T>::kScaled, GL>(gl, g.N), [&](int, double v) { r = v; });
     double sol = group_bcast<G>(r, rM);
     if (!active) continue;
     ++it;
@@ -276,73 +298,104 @@ static size_t lds_for(int N) {
   return wave_lds_bytes<RefArith<T>::kPlain, G>(N);
 }
 
-// Lane groups of 16 keep four boundary lines per wave, groups of 64 one: past the refined length
-// whose four lines fill the LDS, every batch runs on groups of 64.
+// Where the band-boundary lines live. Lane groups of 16 keep four lines per wave, groups of 64
+// one: past the refined length whose four lines fill the LDS, batches run on groups of 64; past
+// the length whose one line fills it, on groups of 16 with their lines in global memory.
 template <typename T>
 static bool g16_fits(int N) { return lds_for<T, 16>(N) <= kMaxLdsBytes; }
-
-// Longest refined column count (n << order) that groups of 64 hold in LDS.
 template <typename T>
-static int64_t max_columns() {
-  int64_t lo = 1, hi = 1 << 20;  // lds_for<T, 64>(lo) fits, lds_for<T, 64>(hi) does not
-  while (hi - lo > 1) {
-    const int64_t mid = (lo + hi) / 2;
-    (lds_for<T, 64>((int)mid) <= kMaxLdsBytes ? lo : hi) = mid;
+static bool g64_fits(int N) { return lds_for<T, 64>(N) <= kMaxLdsBytes; }
+
+// Global boundary lines: at most this much per launch (a launch covers that many waves; longer
+// batches run as several launches on the stream, reusing the block).
+constexpr size_t kGlobalLineBytes = (size_t)4 << 30;
+
+// Launches a 16-lane kernel: launch(integral_constant R, bool_constant GL, w0, waves, lines) with
+// R from the dyadic order. global: lines in global memory, allocated stream-ordered for the call,
+// in launches of as many waves as kGlobalLineBytes holds (at least 1024).
+template <Arith A, class F>
+static hipError_t launch16(int order, int N, int64_t waves, bool global, hipStream_t st,
+                           F &&launch) {
+  auto by_rows = [&](auto glc, int64_t w0, int64_t nw, void *gl) {
+    switch (r16_rows(order)) {
+      case 1: launch(std::integral_constant<int, 1>(), glc, w0, nw, gl); break;
+      case 2: launch(std::integral_constant<int, 2>(), glc, w0, nw, gl); break;
+      case 4: launch(std::integral_constant<int, 4>(), glc, w0, nw, gl); break;
+      default: launch(std::integral_constant<int, 8>(), glc, w0, nw, gl); break;
+    }
+  };
+  if (!global) {
+    by_rows(std::false_type(), 0, waves, nullptr);
+    return hipGetLastError();
   }
-  return lo;
+  const size_t per = r16_line_bytes<A>(N);
+  int64_t chunk = (int64_t)(kGlobalLineBytes / per);
+  if (chunk < 1024) chunk = 1024;
+  if (chunk > waves) chunk = waves;
+  void *gl = nullptr;
+  hipError_t e = hipMallocAsync(&gl, (size_t)chunk * per, st);
+  if (e != hipSuccess) return e;
+  for (int64_t w0 = 0; w0 < waves && e == hipSuccess; w0 += chunk) {
+    by_rows(std::true_type(), w0, waves - w0 < chunk ? waves - w0 : chunk, gl);
+    e = hipGetLastError();
+  }
+  const hipError_t ef = hipFreeAsync(gl, st);
+  return e != hipSuccess ? e : ef;
 }
 
 template <typename T>
 static hipError_t launch_pairwise(const PairwiseArgs<T> &a, hipStream_t st) {
+  constexpr Arith A = RefArith<T>::kPlain;
   const int N = a.n << a.order;
-  if (a.B >= kManyPairs && g16_fits<T>(N)) {
-    const dim3 grid((unsigned)((a.B + 3) / 4));
-    switch (r16_rows(a.order)) {
-      case 1: k_pde_pairwise<T, 16, 1><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
-      case 2: k_pde_pairwise<T, 16, 2><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
-      case 4: k_pde_pairwise<T, 16, 4><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
-      default: k_pde_pairwise<T, 16, 8><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
-    }
-  } else {
-    k_pde_pairwise<T, 64, 1><<<dim3((unsigned)a.B), dim3(kWave), lds_for<T, 64>(N), st>>>(a);
+  const bool f16 = g16_fits<T>(N), f64 = g64_fits<T>(N);
+  if ((a.B >= kManyPairs && f16) || !f64) {
+    const size_t lds = f16 ? lds_for<T, 16>(N) : r16_ring_bytes<A>();
+    return launch16<A>(a.order, N, (a.B + 3) / 4, !f16, st,
+                       [&](auto rc, auto glc, int64_t w0, int64_t nw, void *gl) {
+                         k_pde_pairwise<T, 16, decltype(rc)::value, decltype(glc)::value>
+                             <<<dim3((unsigned)nw), dim3(kWave), lds, st>>>(a, w0, gl);
+                       });
   }
+  k_pde_pairwise<T, 64, 1, false><<<dim3((unsigned)a.B), dim3(kWave), lds_for<T, 64>(N), st>>>(
+      a, 0, nullptr);
   return hipGetLastError();
 }
 
 template <typename T>
 static hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st) {
+  constexpr Arith A = RefArith<T>::kPlain;
   const int N = a.n << a.order;
   const bool tri = a.symmetric && a.off_x == a.off_y && a.bx == a.by;
   const int64_t np = tri ? a.bx * (a.bx + 1) / 2 : a.bx * a.by;
-  if (np >= kManyPairs && g16_fits<T>(N)) {
-    const dim3 grid((unsigned)((np + 3) / 4));
-    const size_t lds = lds_for<T, 16>(N);
-    switch (r16_rows(a.order)) {
-      case 1: k_pde_gram<T, 16, 1><<<grid, dim3(kWave), lds, st>>>(a, np, (int)tri); break;
-      case 2: k_pde_gram<T, 16, 2><<<grid, dim3(kWave), lds, st>>>(a, np, (int)tri); break;
-      case 4: k_pde_gram<T, 16, 4><<<grid, dim3(kWave), lds, st>>>(a, np, (int)tri); break;
-      default: k_pde_gram<T, 16, 8><<<grid, dim3(kWave), lds, st>>>(a, np, (int)tri); break;
-    }
-  } else {
-    k_pde_gram<T, 64, 1><<<dim3((unsigned)np), dim3(kWave), lds_for<T, 64>(N), st>>>(a, np, (int)tri);
+  const bool f16 = g16_fits<T>(N), f64 = g64_fits<T>(N);
+  if ((np >= kManyPairs && f16) || !f64) {
+    const size_t lds = f16 ? lds_for<T, 16>(N) : r16_ring_bytes<A>();
+    return launch16<A>(a.order, N, (np + 3) / 4, !f16, st,
+                       [&](auto rc, auto glc, int64_t w0, int64_t nw, void *gl) {
+                         k_pde_gram<T, 16, decltype(rc)::value, decltype(glc)::value>
+                             <<<dim3((unsigned)nw), dim3(kWave), lds, st>>>(a, np, (int)tri, w0, gl);
+                       });
   }
+  k_pde_gram<T, 64, 1, false><<<dim3((unsigned)np), dim3(kWave), lds_for<T, 64>(N), st>>>(
+      a, np, (int)tri, 0, nullptr);
   return hipGetLastError();
 }
 
 template <typename T>
 static hipError_t launch_robust(const RobustArgs<T> &a, hipStream_t st) {
+  constexpr Arith A = RefArith<T>::kPlain;
   const int N = a.m << a.order;
-  if (a.B >= kManyPairs && g16_fits<T>(N)) {
-    const dim3 grid((unsigned)((a.B + 3) / 4));
-    switch (r16_rows(a.order)) {
-      case 1: k_robust_norm<T, 16, 1><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
-      case 2: k_robust_norm<T, 16, 2><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
-      case 4: k_robust_norm<T, 16, 4><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
-      default: k_robust_norm<T, 16, 8><<<grid, dim3(kWave), lds_for<T, 16>(N), st>>>(a); break;
-    }
-  } else {
-    k_robust_norm<T, 64, 1><<<dim3((unsigned)a.B), dim3(kWave), lds_for<T, 64>(N), st>>>(a);
+  const bool f16 = g16_fits<T>(N), f64 = g64_fits<T>(N);
+  if ((a.B >= kManyPairs && f16) || !f64) {
+    const size_t lds = f16 ? lds_for<T, 16>(N) : r16_ring_bytes<A>();
+    return launch16<A>(a.order, N, (a.B + 3) / 4, !f16, st,
+                       [&](auto rc, auto glc, int64_t w0, int64_t nw, void *gl) {
+                         k_robust_norm<T, 16, decltype(rc)::value, decltype(glc)::value>
+                             <<<dim3((unsigned)nw), dim3(kWave), lds, st>>>(a, w0, gl);
+                       });
   }
+  k_robust_norm<T, 64, 1, false><<<dim3((unsigned)a.B), dim3(kWave), lds_for<T, 64>(N), st>>>(
+      a, 0, nullptr);
   return hipGetLastError();
 }
 
@@ -350,15 +403,19 @@ static hipError_t launch_robust(const RobustArgs<T> &a, hipStream_t st) {
 static int check_grid(int dtype, int64_t m, int64_t n, int order) {
   if (m < 1 || n < 1) return set_error(SIGPDE_EINVAL, "paths need length >= 2 (m, n >= 1 increments)");
   if (order < 0 || order > 12) return set_error(SIGPDE_EINVAL, "dyadic order must be in [0, 12]");
-  if ((m << order) > kMaxRefinedRows)
+  const int64_t M = m << order, N = n << order;
+  if (M > kMaxRefinedRows)
     return set_error(SIGPDE_EINVAL, "refined rows (length-1) << order = %lld exceed %lld",
-                     (long long)(m << order), (long long)kMaxRefinedRows);
-  const int64_t maxc = dtype == SIGPDE_F32 ? max_columns<float>() : max_columns<double>();
-  if ((n << order) > maxc)
+                     (long long)M, (long long)kMaxRefinedRows);
+  if (N > kMaxRefinedCols)
+    return set_error(SIGPDE_EINVAL, "refined columns (length-1) << order = %lld exceed %lld",
+                     (long long)N, (long long)kMaxRefinedCols);
+  // virtual columns of a pair on 16-lane groups (bands x band length) must stay within int
+  if ((M / 16 + 1) * (N + 48) > kMaxVirtualColumns)
     return set_error(SIGPDE_EINVAL,
-                     "refined columns (length-1) << order = %lld exceed %lld, the longest "
-                     "boundary line one wave's LDS holds for this dtype",
-                     (long long)(n << order), (long long)maxc);
+                     "refined grid %lld x %lld too large (bands x band length above %lld)",
+                     (long long)M, (long long)N, (long long)kMaxVirtualColumns);
+  (void)dtype;
   return SIGPDE_OK;
 }
 

@@ -50,9 +50,11 @@ def test_argument_validation_without_device(lib):
     assert L.sigpde_pde_pairwise(0, None, -1, 5, 5, 25, 0, None, None, None, None) == 1
     assert "bad batch" in L.sigpde_last_error().decode()
     fake = ctypes.c_void_p(16)
-    # 5 << 12 = 20480 refined columns: longer than the boundary line one wave's LDS holds
-    assert L.sigpde_pde_pairwise(0, fake, 2, 5, 5, 25, 12, None, None, fake, None) == 1
+    # 2^22 + 1 refined columns; 2^20 x 2^14 refined grid: more virtual columns than int holds
+    assert L.sigpde_pde_pairwise(0, fake, 2, 5, (1 << 22) + 1, 1 << 23, 0, None, None, fake, None) == 1
     assert "refined columns" in L.sigpde_last_error().decode()
+    assert L.sigpde_pde_pairwise(0, fake, 1, 1 << 20, 1 << 14, 1 << 34, 0, None, None, fake, None) == 1
+    assert "too large" in L.sigpde_last_error().decode()
     assert L.sigpde_pde_pairwise(7, fake, 2, 5, 5, 25, 0, None, None, fake, None) == 1
     assert "dtype" in L.sigpde_last_error().decode()
     assert L.sigpde_pde_gram(1, fake, 2, 2, 5, 6, 60, 30, 0, None, None, 1, 0, 0, fake, 2, None) == 1

@@ -1,7 +1,8 @@
-"""GPU parity for long paths: refined column counts past what four boundary lines per wave fit in
-LDS (float64 past 4559, float32 past 9679), where every batch runs on lane groups of 64 (one line
-per wave), up to the longest line one wave holds (17695 float32 / 14927 float64 refined columns);
-long refined row counts (no LDS cost); the fused linear paths up to their 9679 columns; and the
+"""GPU parity for long paths. Refined column counts past what four boundary lines per wave fit in
+LDS (float64 past 4559, float32 past 9679) run on lane groups of 64 (one line per wave in LDS);
+past the longest line one wave holds in LDS (17695 float32 / 14927 float64 refined columns) on
+lane groups of 16 with their lines in global memory, in launches of at most 4 GiB of lines. Also
+long refined row counts (no LDS cost), the fused linear paths up to their 9679 columns, and the
 errors past each limit. Oracle: the CPU restatement on the same increments / paths.
 """
 import numpy as np
@@ -34,6 +35,11 @@ def rand_inc(shape, seed, dtype):
     (torch.float64, 3, 2, 14927, 0),      # the longest float64 line
     (torch.float64, 8, 2, 1200, 3),       # dyadic refinement to 9600 columns
     (torch.float64, 2, 9000, 3, 0),       # long rows: 9000 refined rows, no LDS cost
+    (torch.float32, 3, 2, 17696, 0),      # one column past the float32 LDS line: global lines
+    (torch.float64, 2049, 2, 14928, 0),   # many pairs, global float64 lines
+    (torch.float32, 5, 3, 40000, 0),      # 40000 refined columns
+    (torch.float64, 4, 2, 2500, 3),       # dyadic refinement to 20000 columns, 8 rows per lane
+    (torch.float32, 6, 40, 18000, 0),     # several bands per pair with global lines
 ])
 def test_pairwise_long(dtype, B, m, n, order):
     inc = rand_inc((B, m, n), 11 * m + n + order, dtype) / 4.0 ** order
@@ -45,13 +51,55 @@ def test_pairwise_long(dtype, B, m, n, order):
     assert max_rel(got, ref, floor=1e-3) < 10 * TOL[dtype]
 
 
-@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-def test_pairwise_past_the_line_limit_raises(dtype):
-    n = 14928 if dtype == torch.float64 else 17696
-    inc = torch.zeros((1, 1, n), dtype=dtype, device=DEV)
-    out = torch.zeros(1, dtype=dtype, device=DEV)
+def test_pairwise_global_lines_in_several_launches():
+    """70000 pairs of 17696 refined columns: 17500 waves of 277 KiB of float32 lines, more than the
+    4 GiB one launch takes, so the batch runs as two launches reusing the lines. Every pair reads
+    the same increments (batch stride 0), so every output must equal the one oracle value."""
+    n, B = 17696, 70000
+    inc1 = rand_inc((1, 1, n), 3, torch.float32)
+    ref = O.pde_pairwise(inc1, 0)[0]
+    inc = torch.from_numpy(inc1).to(DEV).expand(B, 1, n)
+    out = torch.zeros(B, dtype=torch.float32, device=DEV)
+    _native.pde_pairwise(inc, 0, out)
+    got = out.cpu().numpy()
+    assert np.all(got == got[0])
+    assert abs(float(got[0]) - float(ref)) <= 1e-5 * abs(float(ref))
+
+
+def test_gram_global_lines():
+    bx, by, m, n = 3, 2, 2, 16000
+    inc = rand_inc((bx, by, m, n), 17, torch.float64)
+    ref = O.pde_gram(inc, 0)
+    out = torch.zeros((bx, by), dtype=torch.float64, device=DEV)
+    _native.pde_gram(torch.from_numpy(inc).to(DEV), 0, out)
+    assert rel_fro(out.cpu().numpy(), ref) < 1e-10
+
+
+def test_robust_norm_global_lines():
+    """Root-find on a 15200 x 15200 refined grid (float64 lines in global memory), two iterations."""
+    B, m, order = 1, 1900, 3
+    rng = np.random.default_rng(13)
+    dx = rng.normal(size=(B, m, 2)) * (1.3 / np.sqrt(m))
+    inc = np.einsum("bsk,btk->bst", dx, dx) / 4.0 ** order
+    lam_ref, k_ref, it_ref = O.norm_factors(inc, order, tol=1e-10, maxit=2)
+    lam = torch.zeros(B, dtype=torch.float64, device=DEV)
+    kn = torch.zeros(B, dtype=torch.float64, device=DEV)
+    iters = torch.zeros(B, dtype=torch.int32, device=DEV)
+    _native.robust_norm(torch.from_numpy(inc).to(DEV), order, lam, kn, None, maxit=2, tol=1e-10,
+                        iters=iters)
+    assert rel_fro(kn.cpu().numpy(), k_ref) < 1e-10
+    assert max_rel(lam.cpu().numpy(), lam_ref, floor=1e-3) < 1e-9
+    assert np.array_equal(iters.cpu().numpy(), it_ref)
+
+
+def test_grid_too_large_raises():
+    out = torch.zeros(1, dtype=torch.float32, device=DEV)
     with pytest.raises(_native.NativeError, match="refined columns"):
-        _native.pde_pairwise(inc, 0, out)
+        inc = torch.zeros((1, 1, (1 << 22) + 1), dtype=torch.float32, device=DEV)
+        _native.pde_pairwise(inc, 0, out)  # 2^22 + 1 refined columns
+    with pytest.raises(_native.NativeError, match="too large"):
+        inc = torch.zeros((1, 1 << 10, 1 << 10), dtype=torch.float32, device=DEV)
+        _native.pde_pairwise(inc, 8, out)  # 2^18 x 2^18 refined grid
 
 
 def test_symmetric_gram_long():

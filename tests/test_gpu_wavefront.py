@@ -125,7 +125,8 @@ def test_bad_arguments_raise():
     inc = torch.zeros((2, 5, 5), dtype=torch.float32, device=DEV)
     out = torch.zeros(2, dtype=torch.float32, device=DEV)
     with pytest.raises(_native.NativeError, match="refined columns"):
-        _native.pde_pairwise(inc, 12, out)  # 5 << 12 = 20480 refined columns: past the LDS lines
+        big = torch.zeros((1, 1, (1 << 22) + 1), dtype=torch.float32, device=DEV)
+        _native.pde_pairwise(big, 0, out[:1])  # 2^22 + 1 refined columns
     with pytest.raises(TypeError):
         _native.pde_pairwise(inc.half(), 0, out)
 
