@@ -53,6 +53,7 @@ MI355X_HBM_GBS = 8000.0
 MI355X_FP64_VALU_TFLOPS = 78.6
 MI355X_FP16_MFMA_TFLOPS = 2500.0
 MI355X_FP32_MFMA_TFLOPS = 157.3
+MI355X_FP64_MFMA_TFLOPS = 78.6
 FLOPS_PER_CELL = 10  # pde_step as written
 # HBM bytes per launch of the fused fp16x2 Gram at the default workload (1024 x 1024 paths, length
 # 256, dim 64, order 0), from rocprofv3 FETCH_SIZE + WRITE_SIZE passes scaled by a 256 MiB copy
@@ -65,7 +66,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--mode", default="fp16x2", choices=["fp16x2", "bf16x3", "fp32", "ref"])
+    p.add_argument("--mode", default="fp16x2", choices=["fp16x2", "bf16x3", "fp32", "ref", "fp64"])
     p.add_argument("--ref-steps", type=int, default=1,
                    help="timed steps of the reference-scheme FP32 path")
     p.add_argument("--paths", type=int, default=1024)
@@ -138,10 +139,12 @@ def fused_roofline(args, X, Y, ms):
             "bf16x3": "k_linear_gram_fused<float, 2, 3> (bf16x3 MFMA increments + double "
                       "wavefront, one launch)",
             "fp32": "k_linear_gram_fused<float, 2, 2> (fp32 MFMA increments + float32-typed "
+                    "wavefront, one launch)",
+            "fp64": "k_linear_gram_fused<float, 2, 4> (fp64 MFMA increments + double "
                     "wavefront, one launch)"}[args.mode]
     traffic = MEASURED_TRAFFIC.get((args.mode, args.paths, args.length, args.dim, args.order))
     alg_bytes = (X.shape[0] * (X.shape[1] - 1) + Y.shape[0] * (Y.shape[1] - 1)) * \
-        ((args.dim + 31) // 32 * 32) * {"fp16x2": 4, "bf16x3": 6, "fp32": 4}[args.mode] + \
+        ((args.dim + 31) // 32 * 32) * {"fp16x2": 4, "bf16x3": 6, "fp32": 4, "fp64": 8}[args.mode] + \
         pairs * X.element_size()
     return {
         "kernel": name,
@@ -157,7 +160,8 @@ def fused_roofline(args, X, Y, ms):
         "cells_per_s": cells / dur_s,
         "increment_gram_tflops": flops_inc / dur_s / 1e12,
         "increment_gram_mfma_frac": flops_inc / dur_s / 1e12 / (
-            MI355X_FP32_MFMA_TFLOPS if args.mode == "fp32" else MI355X_FP16_MFMA_TFLOPS),
+            {"fp32": MI355X_FP32_MFMA_TFLOPS, "fp64": MI355X_FP64_MFMA_TFLOPS}.get(
+                args.mode, MI355X_FP16_MFMA_TFLOPS)),
         "note": "bound: the wavefront's FP64 VALU (the cell update is 8 FP64 instructions; peak = "
                 "MI355X FP64 vector 78.6 TFLOP/s, which is also the FP32 rate without packed math); "
                 "wavefront flops = 10 per refined cell (pde_step as written); the increment Gram "
@@ -320,7 +324,7 @@ def workloads_leg(args, dev):
     K64 = oracle.linear_gram(X.double().cpu().numpy(), Y.double().cpu().numpy(), 2)
     K32ref = oracle.linear_gram(X.cpu().numpy(), Y.cpu().numpy(), 2)  # float32 reference semantics
     g = {}
-    for mode in ("ref", "fp32", "fp16x2", "bf16x3"):
+    for mode in ("ref", "fp32", "fp16x2", "bf16x3", "fp64"):
         k2 = sig.SigPDE(lin, 2, precision=mode)
         k2.gram(X, Y)
         torch.cuda.synchronize()
@@ -349,6 +353,10 @@ def workloads_leg(args, dev):
     k3m.gram(Xr, tol=1e-10)
     torch.cuda.synchronize()
     Krm, ms_gm = _event_ms(lambda: k3m.gram(Xr, tol=1e-10), 3)
+    k3d = sig.RobustSigPDE(lin, 1, precision="fp64")
+    k3d.gram(Xr, tol=1e-10)
+    torch.cuda.synchronize()
+    Krd, ms_gd = _event_ms(lambda: k3d.gram(Xr, tol=1e-10), 3)
     lam = lam.cpu().numpy()
     hist = np.bincount(it, minlength=int(it.max()) + 1)
     out["robust_256_L64_d16_o1_f64"] = {
@@ -357,6 +365,7 @@ def workloads_leg(args, dev):
         "lambda_max_abs_err_fp64": float(np.abs(lam - lam64).max()),
         "gram_rel_err_fp64": _rel(Kr.cpu().numpy(), Kr64),
         "gram_fp16x2_ms": ms_gm, "gram_fp16x2_rel_err_fp64": _rel(Krm.cpu().numpy(), Kr64),
+        "gram_fp64_ms": ms_gd, "gram_fp64_rel_err_fp64": _rel(Krd.cpu().numpy(), Kr64),
         "active_paths": int((lam64 < 1).sum()),
         "iterations_histogram": {str(i): int(c) for i, c in enumerate(hist) if c}}
     return out
@@ -417,6 +426,8 @@ def main():
                        else (K, ms, step_ms))
     Kb, msb, _ = (timed(args, X, Y, "bf16x3", args.steps, 1) if args.mode != "bf16x3" and compare
                   else (K, ms, step_ms))
+    Kd, msd, _ = (timed(args, X, Y, "fp64", 1, 1) if args.mode != "fp64" and compare
+                  and args.dim <= 64 else (K, ms, step_ms))
 
     roof = (fused_roofline(args, X, Y, ms) if args.mode != "ref"
             else dominant_kernel_roofline(args, X, Y))
@@ -433,7 +444,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp16x2": "fp16x2+f64", "bf16x3": "bf16x3+f64", "fp32": "f32",
-                  "ref": "f32"}[args.mode],
+                  "ref": "f32", "fp64": "f64"}[args.mode],
         "data": "synthetic",
         "config": {"workload": f"SigPDE(LinearKernel(), {args.order}).gram(X, Y), X,Y "
                                f"{args.paths} Brownian paths, length {args.length}, dim "
@@ -448,6 +459,9 @@ def main():
         "mixed_bf16x3": {"value": pairs / (msb / 1e3), "unit": "pairs/s", "ms_per_step": msb,
                          "path": "precision='bf16x3': three-part bf16 increments, six bf16 MFMA "
                                  "products, double wavefront"},
+        "fused_fp64": {"value": pairs / (msd / 1e3), "unit": "pairs/s", "ms_per_step": msd,
+                       "path": "precision='fp64': double increments on fp64 MFMA, double "
+                               "wavefront (the reference scheme in float64, fused)"},
         "reference_scheme_fp32": {"value": pairs / (ms_ref / 1e3), "unit": "pairs/s",
                                   "ms_per_step": ms_ref,
                                   "path": "precision='ref': torch increment tiles (point Gram, "
@@ -479,7 +493,7 @@ def main():
         line["rel_err_fp64_full_reference"] = (
             f"all {pairs} entries vs precision='ref' on float64 inputs (GPU, {t64:.1f} s)")
         for name, Ko in (("plain_fp32", K32), ("reference_scheme_fp32", Kref),
-                         ("mixed_bf16x3", Kb)):
+                         ("mixed_bf16x3", Kb), ("fused_fp64", Kd)):
             if Ko is not K:
                 line[name]["rel_err_fp64_full"] = rel_full(Ko)
         del K64

@@ -110,6 +110,8 @@ int sigpde_robust_norm(int dtype, const void *inc, int64_t B, int64_t m, int64_t
  *   precision SIGPDE_PREC_FP32: plain FP32 -- increments rounded to float, products on fp32
  *     MFMA, the wavefront in the reference's float32 typing (c, c^2 in float, the rest in double,
  *     cells stored as float); results in dtype.
+ *   precision SIGPDE_PREC_FP64: increments in double on fp64 MFMA, the wavefront in double (the
+ *     reference scheme in float64, without the increment tensor); d <= 64; results in dtype.
  *   X : (Bx, Lx, d), Y : (By, Ly, d) paths of dtype, C-contiguous; 1 <= d <= 128
  *   sx (Bx,), sy (By,) : optional scales of dtype (c *= sx[i] * sy[j]), or NULL
  *   gram: out[i * ldo + j] = K(x_i, y_j); symmetric != 0 requires Y == X and writes the lower
@@ -119,7 +121,12 @@ int sigpde_robust_norm(int dtype, const void *inc, int64_t B, int64_t m, int64_t
  *     (the device-side increments and per-path scales); same_paths != 0 when Y is X (one split
  *     serves both). Returns 0 for invalid sizes or an unknown precision.
  */
-enum sigpde_precision { SIGPDE_PREC_FP16X2 = 1, SIGPDE_PREC_FP32 = 2, SIGPDE_PREC_BF16X3 = 3 };
+enum sigpde_precision {
+    SIGPDE_PREC_FP16X2 = 1,
+    SIGPDE_PREC_FP32 = 2,
+    SIGPDE_PREC_BF16X3 = 3,
+    SIGPDE_PREC_FP64 = 4
+};
 
 size_t sigpde_linear_workspace_bytes(int precision, int64_t Bx, int64_t Lx, int64_t By, int64_t Ly,
                                      int64_t d, int same_paths);

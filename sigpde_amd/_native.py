@@ -19,7 +19,8 @@ NORM_LOG, NORM_GIVEN = 0, 1
 PREC_FP16X2 = 1
 PREC_FP32 = 2
 PREC_BF16X3 = 3
-PRECISIONS = {"fp16x2": PREC_FP16X2, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+PREC_FP64 = 4
+PRECISIONS = {"fp16x2": PREC_FP16X2, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "fp64": PREC_FP64}
 
 _lib = None
 _lock = threading.Lock()

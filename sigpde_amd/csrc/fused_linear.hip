@@ -18,6 +18,10 @@
 // Precision "fp32" (the plain FP32 fused path): increments rounded to float, products on
 // v_mfma_f32_16x16x4f32 in fp32, scaled in fp32, and the wavefront in the reference's float32
 // typing (Arith::Ref32: c and c^2 in float, the rest in double, cells stored as float).
+//
+// Precision "fp64": increments in double (exact for float inputs), products on
+// v_mfma_f64_16x16x4f64, scaled in double, and the wavefront all in double (Arith::Ref64): the
+// reference scheme in float64 without the increment tensor (dim <= 64).
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -90,6 +94,28 @@ __global__ __launch_bounds__(256) void k_split_f32(const T *__restrict__ X, int 
   }
 }
 
+// fp64 operands: one block per path, dx = x[s+1] - x[s] in double, rows of dpad doubles (zero
+// padded).
+template <typename T>
+__global__ __launch_bounds__(256) void k_split_f64(const T *__restrict__ X, int L, int d, int dpad,
+                                                   double *__restrict__ out) {
+  const int64_t b = blockIdx.x;
+  const T *x = X + b * (int64_t)L * d;
+  const int m = L - 1;
+  double *o = out + b * (int64_t)m * dpad;
+  for (int i = threadIdx.x; i < m * dpad; i += blockDim.x) {
+    const int s = i / dpad, k = i - s * dpad;
+    o[i] = k < d ? (double)x[(s + 1) * d + k] - (double)x[s * d + k] : 0.0;
+  }
+}
+
+// Arithmetic of the wavefront for each producer precision.
+template <int PREC>
+struct FusedArith {
+  static constexpr Arith A =
+      PREC == kPrecFp32 ? Arith::Ref32 : (PREC == kPrecFp64 ? Arith::Ref64 : Arith::Mixed);
+};
+
 // ------------------------------------------------------------------ kernels
 // Gram: a wave takes four x paths (a quad) and streams `ys` y paths through them.
 // Symmetric: only y <= x pairs are written, with their mirror.
@@ -103,7 +129,7 @@ k_linear_gram_fused(GramFusedArgs<TO> a,
                                                           const double *sxd, const double *syd) {
   using Prod = MfmaProducer<KB, 4, 1, PREC>;
   using E = typename Prod::elt;
-  constexpr Arith A = PREC == kPrecFp32 ? Arith::Ref32 : Arith::Mixed;
+  constexpr Arith A = FusedArith<PREC>::A;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int G = 16;
   const int lane = threadIdx.x & 63;
@@ -156,7 +182,7 @@ k_linear_gram_fused(GramFusedArgs<TO> a,
   const int64_t xme = xq * 4 + lane / G;
   {
     const int64_t xc = xme < a.Bx ? xme : a.Bx - 1;
-    P.xsl = PREC != kPrecFp32 ? a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0) : 1.0;
+    P.xsl = Prod::kSplit ? a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0) : 1.0;
     P.pmY = 1.0;
   }
   ring16_solve<A, R>(P, geo, 1.0, smem_raw, [&](int sp, double r) {
@@ -205,9 +231,10 @@ static size_t workspace_layout(int precision, int64_t Bx, int64_t Lx, int64_t By
   };
   const size_t nx = (size_t)Bx * (Lx - 1) * dp, ny = (size_t)By * (Ly - 1) * dp;
   Workspace t{};
-  if (precision == SIGPDE_PREC_FP32) {
-    t.xh = take(nx * 4);
-    t.yh = same ? t.xh : take(ny * 4);
+  if (precision == SIGPDE_PREC_FP32 || precision == SIGPDE_PREC_FP64) {
+    const size_t es = precision == SIGPDE_PREC_FP64 ? 8 : 4;
+    t.xh = take(nx * es);
+    t.yh = same ? t.xh : take(ny * es);
   } else {
     t.xh = take(nx * 2);
     t.xl = take(nx * 2);
@@ -233,6 +260,8 @@ static hipError_t run_split(const T *X, int64_t B, int64_t L, int64_t d, int dp,
   const dim3 grid((unsigned)B), block(256);
   if (PREC == kPrecFp32)
     k_split_f32<T><<<grid, block, 0, st>>>(X, (int)L, (int)d, dp, (float *)h);
+  else if (PREC == kPrecFp64)
+    k_split_f64<T><<<grid, block, 0, st>>>(X, (int)L, (int)d, dp, (double *)h);
   else if (PREC == kPrecBf16x3)
     k_split16<T, __bf16, 3><<<grid, block, 0, st>>>(X, B, (int)L, (int)d, dp, (__bf16 *)h,
                                                     (__bf16 *)mid, (__bf16 *)l, inv);
@@ -284,7 +313,8 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
   a.ldo = ldo;
   const double *sxd = sx ? w.sxd : nullptr;
   const double *syd = sy ? w.syd : nullptr;
-  const size_t shmem = r16_lds_bytes<Arith::Mixed>((int)((Ly - 1) << order));  // = Ref32's
+  // Ref32 and Mixed: float ring and lines; Ref64 (fp64): double
+  const size_t shmem = r16_lds_bytes<FusedArith<PREC>::A>((int)((Ly - 1) << order));
   if (pairwise) {
     return launch_linear_pairwise_fused<T, KB, PREC>(a, sxd, syd, shmem, st);
   } else {
@@ -327,13 +357,22 @@ static hipError_t dispatch_kb(bool pairwise, const void *X, const void *Y, int64
   case K:                                                                                          \
     return launch_fused<T, K, PREC>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,       \
                                     symmetric, out, ldo, w, st);
-  switch (kb) {
-    SIGPDE_KB_CASE(1)
-    SIGPDE_KB_CASE(2)
-    SIGPDE_KB_CASE(3)
-    SIGPDE_KB_CASE(4)
-    default:
-      return hipErrorInvalidValue;
+  if constexpr (PREC == kPrecFp64) {  // dim <= 64
+    switch (kb) {
+      SIGPDE_KB_CASE(1)
+      SIGPDE_KB_CASE(2)
+      default:
+        return hipErrorInvalidValue;
+    }
+  } else {
+    switch (kb) {
+      SIGPDE_KB_CASE(1)
+      SIGPDE_KB_CASE(2)
+      SIGPDE_KB_CASE(3)
+      SIGPDE_KB_CASE(4)
+      default:
+        return hipErrorInvalidValue;
+    }
   }
 #undef SIGPDE_KB_CASE
 }
@@ -350,20 +389,32 @@ static hipError_t dispatch_prec(int precision, bool pairwise, const void *X, con
   if (precision == SIGPDE_PREC_BF16X3)
     return dispatch_kb<T, kPrecBf16x3>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
                                        symmetric, out, ldo, w, st);
+  if (precision == SIGPDE_PREC_FP64)
+    return dispatch_kb<T, kPrecFp64>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
+                                     symmetric, out, ldo, w, st);
   return dispatch_kb<T, kPrecFp16x2>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
                                      symmetric, out, ldo, w, st);
 }
 
 constexpr int64_t kMaxDim = 4 * kKBlock;
+constexpr int64_t kMaxDimFp64 = 2 * kKBlock;  // fp64: x operands of four pairs in registers
 
-// Longest refined column count whose four float boundary lines and ring fit one wave's LDS.
-static int64_t fused_max_columns() {
+// Longest refined column count whose four boundary lines and ring fit one wave's LDS (float line
+// and ring; double for fp64).
+static int64_t fused_max_columns(int precision) {
   int64_t lo = 1, hi = 1 << 20;
   while (hi - lo > 1) {
     const int64_t mid = (lo + hi) / 2;
-    (r16_lds_bytes<Arith::Mixed>((int)mid) <= kMaxLdsBytes ? lo : hi) = mid;
+    const size_t need = precision == SIGPDE_PREC_FP64 ? r16_lds_bytes<Arith::Ref64>((int)mid)
+                                                      : r16_lds_bytes<Arith::Mixed>((int)mid);
+    (need <= kMaxLdsBytes ? lo : hi) = mid;
   }
   return lo;
+}
+
+static bool known_precision(int precision) {
+  return precision == SIGPDE_PREC_FP16X2 || precision == SIGPDE_PREC_FP32 ||
+         precision == SIGPDE_PREC_BF16X3 || precision == SIGPDE_PREC_FP64;
 }
 
 static int check_fused(int dtype, int precision, const void *X, const void *Y, int64_t Bx,
@@ -371,18 +422,19 @@ static int check_fused(int dtype, int precision, const void *X, const void *Y, i
                        void *ws) {
   if (dtype != SIGPDE_F32 && dtype != SIGPDE_F64)
     return set_error(SIGPDE_EINVAL, "linear: unknown dtype %d", dtype);
-  if (precision != SIGPDE_PREC_FP16X2 && precision != SIGPDE_PREC_FP32 &&
-      precision != SIGPDE_PREC_BF16X3)
+  if (!known_precision(precision))
     return set_error(SIGPDE_EINVAL, "linear: unsupported precision mode %d", precision);
   if (Bx < 0 || By < 0 || !X || !Y || !out || !ws)
     return set_error(SIGPDE_EINVAL, "linear: bad batch or null pointer");
   if (Lx < 2 || Ly < 2) return set_error(SIGPDE_EINVAL, "linear: paths need length >= 2");
   if (d < 1 || d > kMaxDim) return set_error(SIGPDE_EINVAL, "linear: dim must be in [1, %d]", (int)kMaxDim);
+  if (precision == SIGPDE_PREC_FP64 && d > kMaxDimFp64)
+    return set_error(SIGPDE_EINVAL, "linear: precision fp64 takes dim <= %d", (int)kMaxDimFp64);
   if (order < 0 || order > 12) return set_error(SIGPDE_EINVAL, "linear: dyadic order must be in [0, 12]");
   if (((Lx - 1) << order) > kMaxRefinedRows)
     return set_error(SIGPDE_EINVAL, "linear: refined rows (Lx-1) << order = %lld exceed %lld",
                      (long long)((Lx - 1) << order), (long long)kMaxRefinedRows);
-  const int64_t maxc = fused_max_columns();
+  const int64_t maxc = fused_max_columns(precision);
   if (((Ly - 1) << order) > maxc)
     return set_error(SIGPDE_EINVAL,
                      "linear: refined columns (Ly-1) << order = %lld exceed %lld, the longest "
@@ -398,9 +450,7 @@ using namespace sigpde;
 extern "C" size_t sigpde_linear_workspace_bytes(int precision, int64_t Bx, int64_t Lx, int64_t By,
                                                int64_t Ly, int64_t d, int same_paths) {
   if (Bx < 0 || By < 0 || Lx < 2 || Ly < 2 || d < 1) return 0;
-  if (precision != SIGPDE_PREC_FP16X2 && precision != SIGPDE_PREC_FP32 &&
-      precision != SIGPDE_PREC_BF16X3)
-    return 0;
+  if (!known_precision(precision)) return 0;
   return workspace_layout(precision, Bx, Lx, By, Ly, d, same_paths, nullptr, nullptr);
 }
 

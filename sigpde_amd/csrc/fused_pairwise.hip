@@ -13,7 +13,8 @@ __global__ __launch_bounds__(64) void k_linear_pairwise_fused(GramFusedArgs<TO> 
                                                               const double *syd) {
   using Prod = MfmaProducer<KB, 4, 4, PREC>;
   using E = typename Prod::elt;
-  constexpr Arith A = PREC == kPrecFp32 ? Arith::Ref32 : Arith::Mixed;
+  constexpr Arith A =
+      PREC == kPrecFp32 ? Arith::Ref32 : (PREC == kPrecFp64 ? Arith::Ref64 : Arith::Mixed);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int G = 16;
   const int lane = threadIdx.x & 63;
@@ -44,7 +45,7 @@ __global__ __launch_bounds__(64) void k_linear_pairwise_fused(GramFusedArgs<TO> 
   {
     const int64_t xc = bme < a.Bx ? bme : a.Bx - 1;
     P.ylane = bme < a.By ? bme : a.By - 1;
-    P.xsl = PREC != kPrecFp32 ? a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0) : 1.0;
+    P.xsl = Prod::kSplit ? a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0) : 1.0;
     P.pmY = 1.0;  // the y scale is read when the pair's first chunk is multiplied
   }
   ring16_solve<A, R>(P, geo, 1.0, smem_raw, [&](int, double r) {
@@ -81,5 +82,13 @@ SIGPDE_PAIRWISE_INST(double, 2)
 SIGPDE_PAIRWISE_INST(double, 3)
 SIGPDE_PAIRWISE_INST(double, 4)
 #undef SIGPDE_PAIRWISE_INST
+template hipError_t launch_linear_pairwise_fused<float, 1, kPrecFp64>(
+    const GramFusedArgs<float> &, const double *, const double *, size_t, hipStream_t);
+template hipError_t launch_linear_pairwise_fused<float, 2, kPrecFp64>(
+    const GramFusedArgs<float> &, const double *, const double *, size_t, hipStream_t);
+template hipError_t launch_linear_pairwise_fused<double, 1, kPrecFp64>(
+    const GramFusedArgs<double> &, const double *, const double *, size_t, hipStream_t);
+template hipError_t launch_linear_pairwise_fused<double, 2, kPrecFp64>(
+    const GramFusedArgs<double> &, const double *, const double *, size_t, hipStream_t);
 
 }  // namespace sigpde

@@ -14,6 +14,8 @@ namespace sigpde {
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float float4v __attribute__((ext_vector_type(4)));
+typedef double double2v __attribute__((ext_vector_type(2)));
+typedef double double4v __attribute__((ext_vector_type(4)));
 
 constexpr int kKBlock = 32;  // dimension padding unit (K of one v_mfma_f32_16x16x32_{f16,bf16})
 
@@ -21,6 +23,7 @@ constexpr int kKBlock = 32;  // dimension padding unit (K of one v_mfma_f32_16x1
 constexpr int kPrecFp16x2 = 1;  // per-path-scaled fp16 hi + lo, 3 f16 MFMAs per 32-wide K block
 constexpr int kPrecFp32 = 2;    // fp32 increments, v_mfma_f32_16x16x4f32
 constexpr int kPrecBf16x3 = 3;  // bf16 hi + mid + lo, 6 bf16 MFMAs per 32-wide K block
+constexpr int kPrecFp64 = 4;    // double increments, v_mfma_f64_16x16x4f64
 
 // ------------------------------------------------------------------ MFMA producer (G = 16)
 // Lane group g (16 lanes) solves pairs (x path xa[g], y path ybase[ys(g)] + sp * ystep) of the
@@ -38,6 +41,10 @@ constexpr int kPrecBf16x3 = 3;  // bf16 hi + mid + lo, 6 bf16 MFMAs per 32-wide 
 //   fp32:   lane l supplies elements (l/16)*dpad/4 + j, j < dpad/4, one per MFMA (any split of
 //           K works as long as A and B use the same one); the tiles are scaled by
 //           kscale/4^order (x_scale y_scale) in fp32 and the consumer runs Ref32.
+//   fp64:   as fp32 in double on v_mfma_f64_16x16x4f64, scaled in double, consumer Ref64. That
+//           instruction leaves D[h + 4e][c] in lane (c, h) element e (f32: D[4h + e][c]); lane
+//           (c, h) therefore supplies A row 4(c & 3) + (c >> 2) instead of row c, which permutes
+//           D's rows so that element e is again row 4h + e.
 // Pipeline of period p: at its start, multiply chunk p+1 and fetch chunk p+2's B operands; between
 // its consumer groups 2 and 3, write chunk p+1's tiles to the skewed ring of ring16.h (the
 // accumulators never live across periods).
@@ -52,12 +59,21 @@ struct MfmaProducer {
   __device__ static constexpr int ys(int g) { return NY == 4 ? g : 0; }
   static constexpr bool kF16 = PREC == kPrecFp16x2;
   static constexpr bool kB3 = PREC == kPrecBf16x3;
+  static constexpr bool kF64 = PREC == kPrecFp64;
   static constexpr bool kSplit = kF16 || kB3;     // 16-bit parts, pair scale applied by Mixed
-  static constexpr int NV = kSplit ? KB : 2 * KB;  // operand vectors per lane (8 x 16 bit or float4)
-  typedef typename std::conditional<kF16, half8,
-                                    typename std::conditional<kB3, bf16x8, float4v>::type>::type vec;
-  typedef typename std::conditional<kF16, _Float16,
-                                    typename std::conditional<kB3, __bf16, float>::type>::type elt;
+  static constexpr int kVE = kSplit ? 8 : (kF64 ? 2 : 4);  // elements per operand vector
+  static constexpr int NV = KB * kKBlock / 4 / kVE;        // operand vectors per lane and set
+  typedef typename std::conditional<
+      kF16, half8,
+      typename std::conditional<kB3, bf16x8,
+                                typename std::conditional<kF64, double2v, float4v>::type>::type>::type
+      vec;
+  typedef typename std::conditional<
+      kF16, _Float16,
+      typename std::conditional<kB3, __bf16,
+                                typename std::conditional<kF64, double, float>::type>::type>::type elt;
+  typedef typename std::conditional<kF64, double, float>::type RT;        // ring element
+  typedef typename std::conditional<kF64, double4v, float4v>::type acc_t;  // one lane's tile part
   const elt *xh, *xl, *yh, *yl;          // fp32: xh / yh only
   const elt *xm, *ym;                    // bf16x3: middle parts
   int dpad;
@@ -76,12 +92,12 @@ struct MfmaProducer {
 
   vec Ah[NX][NV], Al[NX][kSplit ? NV : 1], Am[NX][kB3 ? NV : 1];  // A operands of the band
   vec Bh[NY][NV], Bl[NY][kSplit ? NV : 1], Bm[NY][kB3 ? NV : 1];  // next chunk's B operands
-  float4v acc[NP];
-  float mult[NP];                        // fp32: scale of the tiles in acc
-  float pm[NP];                          // fp32: scale of the pair being multiplied
+  acc_t acc[NP];
+  RT mult[NP];                           // fp32 / fp64: scale of the tiles in acc
+  RT pm[NP];                             // fp32 / fp64: scale of the pair being multiplied
   int64_t yrow[NY];                      // first B row (in elements) of the pair being fetched
   Cursor<G> cB, cC;                      // next chunk to fetch B for / to multiply
-  Ring16Writer<float> W;                 // chunk writes into the skewed ring
+  Ring16Writer<RT> W;                    // chunk writes into the skewed ring
 
   __device__ __forceinline__ double lane_scale() const { return xsl * pmY; }
 
@@ -103,14 +119,16 @@ struct MfmaProducer {
         dl[v] = *reinterpret_cast<const vec *>(pl + lo + v * kKBlock);
         if constexpr (kB3) dm[v] = *reinterpret_cast<const vec *>(pm + lo + v * kKBlock);
       } else {
-        dh[v] = *reinterpret_cast<const vec *>(ph + lo + 4 * v);
+        dh[v] = *reinterpret_cast<const vec *>(ph + lo + kVE * v);
       }
     }
   }
 
   __device__ __forceinline__ void fetchA(const PairGeom<G> &geo, int band) {
     const int lane = threadIdx.x & 63;
-    int ro = (band * G * geo.R + geo.R * (lane & 15)) >> geo.order;
+    const int c = lane & 15;
+    const int rl = kF64 ? 4 * (c & 3) + (c >> 2) : c;  // fp64: row permutation (see above)
+    int ro = (band * G * geo.R + geo.R * rl) >> geo.order;
     ro = ro > m - 1 ? m - 1 : ro;
     const uint32_t lo = (uint32_t)(ro * dpad) + kslice();
 #pragma unroll
@@ -154,14 +172,14 @@ struct MfmaProducer {
           double mm = kmult;
           if (sx) mm *= sx[xp];
           if (sy) mm *= sy[yp];
-          pm[g] = (float)mm;
+          pm[g] = (RT)mm;
         }
       }
     }
 #pragma unroll
     for (int g = 0; g < NP; ++g) {
       const int a = xs(g), q = ys(g);
-      float4v c = {0.f, 0.f, 0.f, 0.f};
+      acc_t c = {0, 0, 0, 0};
       if constexpr (kF16) {
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) {
@@ -187,11 +205,18 @@ struct MfmaProducer {
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
           c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[a][kb], Bh[q][kb], c, 0, 0, 0);
+      } else if constexpr (kF64) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+          for (int e = 0; e < kVE; ++e)
+            c = __builtin_amdgcn_mfma_f64_16x16x4f64(Ah[a][v][e], Bh[q][v][e], c, 0, 0, 0);
+        mult[g] = pm[g];
       } else {
 #pragma unroll
         for (int v = 0; v < NV; ++v)
 #pragma unroll
-          for (int e = 0; e < 4; ++e)
+          for (int e = 0; e < kVE; ++e)
             c = __builtin_amdgcn_mfma_f32_16x16x4f32(Ah[a][v][e], Bh[q][v][e], c, 0, 0, 0);
         mult[g] = pm[g];
       }
@@ -201,8 +226,8 @@ struct MfmaProducer {
     if (more && cC.jv0 == 0) fetchA(geo, cC.band);
   }
   template <int PAR>
-  __device__ __forceinline__ void store(float *ring) {
-    float v[NP][4];
+  __device__ __forceinline__ void store(RT *ring) {
+    RT v[NP][4];
 #pragma unroll
     for (int g = 0; g < NP; ++g)
 #pragma unroll
@@ -210,7 +235,7 @@ struct MfmaProducer {
     W.template store<PAR>(ring, v);
   }
 
-  __device__ __forceinline__ void prologue(float *ring, const PairGeom<G> &geo) {
+  __device__ __forceinline__ void prologue(RT *ring, const PairGeom<G> &geo) {
     const int nchunks = geo.columns() / kChunk;
     W.init();
     cB.start(geo, 0);
@@ -222,14 +247,14 @@ struct MfmaProducer {
     if (nchunks > 1) fetchB(geo);
   }
   // start of period p: multiply chunk p+1, fetch chunk p+2's B operands
-  __device__ __forceinline__ void early(float *, const PairGeom<G> &geo, int per) {
+  __device__ __forceinline__ void early(RT *, const PairGeom<G> &geo, int per) {
     const int nchunks = geo.columns() / kChunk;
     if (per + 1 < nchunks) compute(geo, per + 2 < nchunks);
     if (per + 2 < nchunks) fetchB(geo);
   }
   // between consumer groups 2 and 3 of period p: write chunk p+1 (ring parity PAR)
   template <int PAR>
-  __device__ __forceinline__ void period(float *ring, const PairGeom<G> &geo, int per) {
+  __device__ __forceinline__ void period(RT *ring, const PairGeom<G> &geo, int per) {
     const int nchunks = geo.columns() / kChunk;
     if (per + 1 < nchunks) store<PAR>(ring);
   }
@@ -238,7 +263,8 @@ struct MfmaProducer {
 // Arguments of the fused kernels (x / y operands from the split prepass).
 template <typename TO>
 struct GramFusedArgs {
-  const void *xh, *xl, *yh, *yl;  // fp16x2: _Float16 hi / lo rows; fp32: float rows in xh / yh
+  const void *xh, *xl, *yh, *yl;  // fp16x2: _Float16 hi / lo rows; fp32 / fp64: float / double
+                                  // rows in xh / yh
   const void *xm, *ym;            // bf16x3: middle parts (hi / lo in xh / xl)
   const float *invx, *invy;       // fp16x2, bf16x3: per-path 2^-e
   int64_t Bx, By;

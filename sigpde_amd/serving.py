@@ -19,9 +19,9 @@ sigpde/torch/kernel.py:26-301 API) for a stream of independent requests:
     cannot fit alone fails with MemoryError.
   * Streams: batches are issued round-robin on `streams` HIP streams of default priority;
     completion is tracked with events.
-  * HIP graphs (use_graphs=True): fused pairwise batches (precision "fp16x2"/"bf16x3"/"fp32") are
-    captured once per (shape class, padded batch size) and replayed with the inputs copied into
-    the graph's static buffers.
+  * HIP graphs (use_graphs=True): fused pairwise batches (precision "fp16x2", "bf16x3", "fp32" or
+    "fp64") are captured once per (shape class, padded batch size) and replayed with the inputs
+    copied into the graph's static buffers.
 
 `python -m sigpde_amd.serving` runs a synthetic request mix and prints one JSON line.
 """
@@ -488,7 +488,8 @@ def main(argv=None):
                    help="Poisson arrivals per second (default: all queued at t=0, saturation)")
     p.add_argument("--budget-gb", type=float, default=64.0)
     p.add_argument("--streams", type=int, default=4)
-    p.add_argument("--precision", default="ref", choices=["ref", "fp16x2", "bf16x3", "fp32"])
+    p.add_argument("--precision", default="ref",
+                   choices=["ref", "fp16x2", "bf16x3", "fp32", "fp64"])
     p.add_argument("--graphs", action="store_true")
     p.add_argument("--max-length", type=int, default=512)
     args = p.parse_args(argv)

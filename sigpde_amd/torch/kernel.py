@@ -97,7 +97,7 @@ def _scale_vector(s, batch: int, like: torch.Tensor) -> torch.Tensor:
     return s.contiguous()
 
 
-PRECISIONS = ("ref", "fp16x2", "bf16x3", "fp32")
+PRECISIONS = ("ref", "fp16x2", "bf16x3", "fp32", "fp64")
 
 
 def _fused_scale(static_kernel, precision: str):
@@ -127,7 +127,10 @@ class SigPDE:
                   and six bf16 MFMA products (the terms of magnitude >= 2^-16 of hi*hi);
       "fp32"   -- linear kernel only, plain FP32 fused: increments rounded to float, <dx, dy> on
                   fp32 MFMA, the wavefront in the reference's float32 typing (no increment
-                  tensor in HBM, as for "fp16x2").
+                  tensor in HBM, as for "fp16x2");
+      "fp64"   -- linear kernel only, dim <= 64: increments <dx, dy> in double on fp64 MFMA, the
+                  wavefront in double (the reference scheme in float64 without the increment
+                  tensor; results in the inputs' dtype).
     """
 
     def __init__(self, static_kernel, dyadic_order: int = 0, precision: str = "ref"):
@@ -231,9 +234,9 @@ class RobustSigPDE:
     """Robust (normalised) signature kernel (stojl/SigPDE sigpde/torch/kernel.py:174-301).
 
     precision (an addition to the reference's signature) selects how the final scaled solves of
-    pairwise / gram run, with the same meaning as for SigPDE ("ref", "fp16x2", "bf16x3", "fp32"; the fused
-    modes need the linear static kernel). The normalisation root-find always runs the
-    reference's arithmetic on the increments of the inputs' dtype.
+    pairwise / gram run, with the same meaning as for SigPDE ("ref", "fp16x2", "bf16x3", "fp32",
+    "fp64"; the fused modes need the linear static kernel). The normalisation root-find always
+    runs the reference's arithmetic on the increments of the inputs' dtype.
     """
 
     def __init__(self, static_kernel, dyadic_order: int = 0, normalizer: Optional[Callable] = None,

@@ -6,7 +6,10 @@ inputs upcast:
   precision="bf16x3" (three bf16 parts, six bf16 MFMA products, double wavefront): same bars as
     fp16x2;
   precision="fp32" (plain FP32: fp32 increments on MFMA, the reference's float32 wavefront
-    typing): float32 storage of every cell bounds it; bar 1e-4 Frobenius / 1e-3 per entry.
+    typing): float32 storage of every cell bounds it; bar 1e-4 Frobenius / 1e-3 per entry;
+  precision="fp64" (double increments on fp64 MFMA, double wavefront, dim <= 64): the float32
+    rounding of the result bounds it on float32 inputs (bar 2e-7); on float64 inputs it is the
+    float64 reference scheme (bar 1e-12).
 """
 import numpy as np
 import pytest
@@ -15,10 +18,11 @@ import torch
 import oracle as O
 import sigpde_amd.torch as sig
 from helpers import brownian_np, rel_fro
+from sigpde_amd import _native
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda"
-BARS = {"fp16x2": 1e-5, "bf16x3": 1e-5, "fp32": 1e-4}
+BARS = {"fp16x2": 1e-5, "bf16x3": 1e-5, "fp32": 1e-4, "fp64": 2e-7}
 PRECS = list(BARS)
 
 
@@ -46,6 +50,10 @@ def test_fused_gram_vs_f64(L, d, order, var, prec):
     x = brownian_np(9, L, d, var, 5, np.float32)
     y = brownian_np(7, L, d, var, 6, np.float32)
     k = sig.SigPDE(sig.kernels.LinearKernel(), order, precision=prec)
+    if prec == "fp64" and d > 64:
+        with pytest.raises(_native.NativeError, match="dim <= 64"):
+            k.gram(cuda(x), cuda(y))
+        return
     got = k.gram(cuda(x), cuda(y)).cpu().numpy().astype(np.float64)
     ref = O.linear_gram(x.astype(np.float64), y.astype(np.float64), order)
     check(got, ref, prec)
@@ -84,6 +92,8 @@ def test_fused_gram_float64_inputs(prec):
     got = k.gram(cuda(x), cuda(y))
     assert got.dtype == torch.float64
     check(got.cpu().numpy(), O.linear_gram(x, y, 1), prec)
+    if prec == "fp64":
+        assert rel_fro(got.cpu().numpy(), O.linear_gram(x, y, 1)) < 1e-12
 
 
 @pytest.mark.parametrize("prec", PRECS)
@@ -129,7 +139,7 @@ def test_fused_rejects_nonlinear_kernel(prec):
         sig.SigPDE(sig.kernels.RBFKernel(), 0, precision=prec)
 
 
-@pytest.mark.parametrize("prec", ["fp16x2", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp16x2", "bf16x3", "fp64"])
 def test_fused_full_1024x1024_gram_vs_gpu_float64(prec):
     """Every entry of the 1024 x 1024 Gram (length 256, dim 64, order 0) against the float64 GPU
     path (precision="ref" on the float64 inputs: the reference scheme in float64, pinned to the CPU

@@ -129,16 +129,21 @@ def test_robust_norm_long():
     assert max_rel(lam.cpu().numpy(), lam_ref, floor=1e-3) < 1e-9
 
 
-@pytest.mark.parametrize("prec", ["fp16x2", "bf16x3", "fp32"])
+@pytest.mark.parametrize("prec", ["fp16x2", "bf16x3", "fp32", "fp64"])
 @pytest.mark.parametrize("Lx,Ly", [(64, 9680), (12000, 40)])
 def test_fused_gram_long(prec, Lx, Ly):
-    """Fused linear Gram with 9679 refined columns (its longest line) and with 11999 refined rows."""
+    """Fused linear Gram with 9679 refined columns (the longest float line; fp64, whose lines and
+    ring are double, takes 4559 and must refuse it) and with 11999 refined rows."""
     X = brownian_np(3, Lx, 8, 1.0 / max(Lx, Ly), 21, np.float32)
     Y = brownian_np(2, Ly, 8, 1.0 / max(Lx, Ly), 22, np.float32)
+    k = sig.SigPDE(sig.kernels.LinearKernel(), 0, precision=prec)
+    if prec == "fp64" and Ly - 1 > 4559:
+        with pytest.raises(_native.NativeError, match="refined columns"):
+            k.gram(torch.from_numpy(X).to(DEV), torch.from_numpy(Y).to(DEV))
+        Y = Y[:, :4560]
     ref = O.linear_gram(X.astype(np.float64), Y.astype(np.float64), 0)
-    K = sig.SigPDE(sig.kernels.LinearKernel(), 0, precision=prec).gram(
-        torch.from_numpy(X).to(DEV), torch.from_numpy(Y).to(DEV))
-    bar = 1e-4 if prec == "fp32" else 1e-5
+    K = k.gram(torch.from_numpy(X).to(DEV), torch.from_numpy(np.ascontiguousarray(Y)).to(DEV))
+    bar = {"fp32": 1e-4, "fp64": 2e-7}.get(prec, 1e-5)
     assert rel_fro(K.double().cpu().numpy(), ref) < bar
 
 

@@ -130,7 +130,7 @@ def test_budget_serialises_and_rejects():
     assert isinstance(big.error, MemoryError)
 
 
-@pytest.mark.parametrize("prec", ["fp16x2", "fp32"])
+@pytest.mark.parametrize("prec", ["fp16x2", "fp32", "fp64"])
 def test_graph_replays_equal_eager(prec):
     k = sig.kernels.LinearKernel()
     srv = S.Server(DEV, use_graphs=True, streams=1)
