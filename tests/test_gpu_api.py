@@ -22,7 +22,7 @@ def cuda(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
 
 
-def test_cfg1_pairwise_f64():
+def test_pairwise_f64_length32_order1():
     """SigPDE(LinearKernel(), 1).pairwise(x, y): batch 8, length 32, dim 4, float64."""
     x = brownian_np(8, 32, 4, 1 / 32, 0)
     y = brownian_np(8, 32, 4, 1 / 32, 1)
@@ -59,8 +59,8 @@ def test_gram_f64_tiles(max_batch):
     assert np.array_equal(gs, gs.T)
 
 
-def test_cfg2_gram_f32_quantized():
-    """cfg2 shape (length 100, dim 8, order 2) on a 24 x 20 block, float32 reference semantics."""
+def test_gram_f32_length100_order2_quantized():
+    """Length 100, dim 8, order 2 on a 24 x 20 block, float32 reference semantics."""
     x = quantized_paths(24, 100, 8, 7)
     y = quantized_paths(20, 100, 8, 8)
     got = sig.SigPDE(sig.kernels.LinearKernel(), 2).gram(cuda(x), cuda(y)).cpu().numpy()
@@ -69,7 +69,7 @@ def test_cfg2_gram_f32_quantized():
     assert max_rel(got, ref, floor=1e-2 * np.abs(ref).max()) < 1e-5
 
 
-def test_cfg2_gram_f32_brownian_vs_f64():
+def test_gram_f32_length100_order2_brownian_vs_f64():
     x = brownian_np(16, 100, 8, 1 / 100, 2, np.float32)
     y = brownian_np(16, 100, 8, 1 / 100, 3, np.float32)
     got = sig.SigPDE(sig.kernels.LinearKernel(), 2).gram(cuda(x), cuda(y)).cpu().numpy()
@@ -93,7 +93,7 @@ def test_nonlinear_static_kernels(kind, param):
 
 
 def test_robust_normalization_f64():
-    """cfg3 shape (length 64, dim 16, order 1, N(0, 9/64) increments, tol 1e-10) on 32 paths."""
+    """Length 64, dim 16, order 1, N(0, 9/64) increments, tol 1e-10, on 32 paths."""
     x = brownian_np(32, 64, 16, 9 / 64, 4)
     rk = sig.RobustSigPDE(sig.kernels.LinearKernel(), 1)
     lam = rk.normalization(cuda(x), tol=1e-10).cpu().numpy()

@@ -39,9 +39,9 @@ def check(got, ref, prec="fp16x2"):
 
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("L,d,order,var", [
-    (256, 64, 0, 1 / 256),   # cfg4 shape
-    (100, 8, 2, 1 / 100),    # cfg2 shape (d padded to 32)
-    (64, 16, 1, 9 / 64),     # cfg3 shape, large K values
+    (256, 64, 0, 1 / 256),   # the 1024 x 1024 workload's path shape
+    (100, 8, 2, 1 / 100),    # length 100, dim 8, order 2 (d padded to 32)
+    (64, 16, 1, 9 / 64),     # length 64, dim 16, order 1, large K values
     (33, 4, 3, 1 / 33),      # order 3
     (20, 100, 0, 1 / 20),    # d = 100: four K blocks
     (513, 4, 0, 1 / 512),    # 512 increments: 32 bands of 16 rows

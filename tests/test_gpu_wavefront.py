@@ -27,13 +27,13 @@ def rand_inc(shape, scale, seed, dtype):
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("B,m,n,order", [
-    (8, 31, 31, 1),      # cfg1 grid
+    (8, 31, 31, 1),      # 8 pairs of length 32, order 1
     (5, 1, 1, 0),        # single cell
     (3, 7, 13, 0),       # m != n, tiny
     (4, 13, 7, 2),
-    (6, 63, 63, 1),      # cfg3 grid
-    (3, 99, 99, 2),      # cfg2 grid: 396 rows, one band of 7 rows per lane
-    (2, 255, 255, 0),    # cfg4 grid
+    (6, 63, 63, 1),      # length 64, order 1
+    (3, 99, 99, 2),      # length 100, order 2: 396 rows, one band of 7 rows per lane
+    (2, 255, 255, 0),    # length 256, order 0
     (2, 1100, 40, 0),    # two bands (rows > 64 * 16)
     (2, 127, 90, 3),     # order 3, 1016 rows
     (1, 511, 20, 3),     # 4088 rows: four bands, the largest refined side
