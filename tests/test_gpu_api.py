@@ -166,3 +166,34 @@ def test_api_validation():
         k.pairwise(torch.zeros((2, 5, 3), device=DEV, dtype=torch.float16))
     out = k.pairwise(torch.zeros((0, 5, 3), device=DEV))
     assert out.shape == (0,)
+
+
+def _golden(name):
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)
+    return np.load(path, allow_pickle=False)
+
+
+def test_golden_fixtures_through_the_api():
+    """The committed fixtures (tests/golden: seeded inputs and the oracle's outputs) through the
+    public API: float64 pairwise and robust normalisation / Gram at 1e-10 / 1e-9, the float32 Gram
+    in the reference scheme against its float64 value (the float32 point-Gram rounding: 5e-5) and
+    on the fused paths at their bars."""
+    lin = sig.kernels.LinearKernel()
+    g = _golden("pairwise_f64_L32_d4_o1.npz")
+    got = sig.SigPDE(lin, 1).pairwise(cuda(g["x"]), cuda(g["y"])).cpu().numpy()
+    assert rel_fro(got, g["K64"]) < 1e-10
+    got = sig.SigPDE(lin, 1, precision="fp64").pairwise(cuda(g["x"]), cuda(g["y"])).cpu().numpy()
+    assert rel_fro(got, g["K64"]) < 1e-12
+    g = _golden("gram_f32_L100_d8_o2.npz")
+    X, Y = cuda(g["X"]), cuda(g["Y"])
+    for prec, bar in (("ref", 5e-5), ("fp32", 1e-4), ("fp16x2", 1e-5), ("bf16x3", 1e-5),
+                      ("fp64", 2e-7)):
+        got = sig.SigPDE(lin, 2, precision=prec).gram(X, Y).double().cpu().numpy()
+        assert rel_fro(got, g["K64"]) < bar, prec
+    g = _golden("robust_f64_L64_d16_o1.npz")
+    rk = sig.RobustSigPDE(lin, 1)
+    lam = rk.normalization(cuda(g["X"]), tol=1e-10).cpu().numpy()
+    assert np.max(np.abs(lam - g["lam"])) < 1e-9
+    K = rk.gram(cuda(g["X"]), tol=1e-10).cpu().numpy()
+    assert rel_fro(K, g["K"]) < 1e-9

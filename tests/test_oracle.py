@@ -11,7 +11,9 @@
   * the increment second difference equals <dx_s, dy_t> / 4^o for the linear kernel;
   * static kernels against direct loops;
   * Chandrupatla roots against scipy's brentq and bisection, the first-iterate bookkeeping, and the
-    normalisation special cases (NaN -> inf, K < 1 -> 0).
+    normalisation special cases (NaN -> inf, K < 1 -> 0);
+  * the golden fixtures in tests/golden (the oracle's outputs on seeded inputs of the smaller
+    workloads, written by tests/golden/make_fixtures.py) are reproduced.
 """
 import math
 
@@ -267,3 +269,26 @@ def test_norm_factors_float32_close_to_float64():
     l64 = O.norm_factors(inc, 1, tol=1e-10)[0]
     l32 = O.norm_factors(inc.astype(np.float32), 1, tol=1e-10)[0]
     assert np.allclose(l32, l64, rtol=1e-5)
+
+
+# ----------------------------------------------------------------------------- golden fixtures
+def _golden(name):
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)
+    return np.load(path, allow_pickle=False)
+
+
+def test_oracle_reproduces_golden_fixtures():
+    g = _golden("pairwise_f64_L32_d4_o1.npz")
+    assert rel_fro(O.sig_pairwise(g["x"], g["y"], 1), g["K64"]) < 1e-14
+    g = _golden("gram_f32_L100_d8_o2.npz")
+    X, Y = g["X"], g["Y"]
+    assert rel_fro(O.linear_gram(X.astype(np.float64), Y.astype(np.float64), 2), g["K64"]) < 1e-14
+    assert rel_fro(O.linear_gram(X, Y, 2).astype(np.float64), g["K32"].astype(np.float64)) < 1e-7
+    g = _golden("robust_f64_L64_d16_o1.npz")
+    inc = O.second_difference(O.static_pairwise("linear", g["X"], g["X"]), 1)
+    lam, knorm, iters = O.norm_factors(inc, 1, tol=1e-10, maxit=100)
+    assert np.max(np.abs(lam - g["lam"])) < 1e-13
+    assert rel_fro(knorm, g["knorm"]) < 1e-14
+    assert np.array_equal(iters, g["iters"])
+    assert rel_fro(O.linear_gram(g["X"], g["X"], 1, sx=lam, sy=lam), g["K"]) < 1e-13
