@@ -151,6 +151,42 @@ def test_robust_norm_log(dtype):
         assert np.array_equal(iters.cpu().numpy(), it_ref)
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_nonfinite_inputs_follow_the_reference(dtype):
+    """NaN increments propagate to NaN solves; in the root-find K(x,x) = NaN becomes +inf
+    (sigpde/torch/kernel.py:183, sigpde/cuda_pairwise_kernels.py:132), the target is then the
+    normaliser's limit 2, and lambda follows the same iterate sequence as the oracle. Huge increments
+    overflow float32 to +inf the same way."""
+    B, m, order = 4, 9, 1
+    rng = np.random.default_rng(31)
+    dx = rng.normal(size=(B, m, 3)) * 0.5
+    inc = (np.einsum("bsk,btk->bst", dx, dx) / 4.0 ** order).astype(NP[dtype])
+    inc[1, 3, 4] = np.nan
+    if dtype == torch.float32:
+        inc[2] *= 1e4  # K(x,x) overflows float32
+    t = torch.from_numpy(inc).to(DEV)
+    out = torch.zeros(B, dtype=dtype, device=DEV)
+    _native.pde_pairwise(t, order, out)
+    ref = O.pde_pairwise(inc, order)
+    got = out.cpu().numpy()
+    assert np.isnan(got[1]) and np.isnan(ref[1])
+    fin = np.isfinite(ref)
+    assert np.array_equal(np.isfinite(got), fin)
+    assert rel_fro(got[fin], ref[fin]) < TOL[dtype]
+    lam_ref, k_ref, it_ref = O.norm_factors(inc, order, tol=1e-10, maxit=30)
+    lam = torch.zeros(B, dtype=dtype, device=DEV)
+    kn = torch.zeros(B, dtype=dtype, device=DEV)
+    iters = torch.zeros(B, dtype=torch.int32, device=DEV)
+    _native.robust_norm(t, order, lam, kn, None, maxit=30, tol=1e-10, iters=iters)
+    kn, lam = kn.cpu().numpy(), lam.cpu().numpy()
+    assert np.isinf(kn[1]) and np.isinf(k_ref[1])
+    assert np.array_equal(np.isinf(kn), np.isinf(k_ref))
+    tol_lam = 1e-9 if dtype == torch.float64 else 1e-5
+    assert max_rel(lam, lam_ref, floor=1e-3) < tol_lam
+    if dtype == torch.float64:
+        assert np.array_equal(iters.cpu().numpy(), it_ref)
+
+
 def test_robust_norm_given_target():
     B, m, order = 8, 20, 0
     rng = np.random.default_rng(5)
