@@ -293,7 +293,8 @@ def workloads_leg(args, dev):
       pairwise  SigPDE(LinearKernel(), 1).pairwise(x, y), float64, 8 pairs, length 32, dim 4
                 (latency-bound: per-call wall time p50 / p99, synchronised on return)
       gram      SigPDE(LinearKernel(), 2).gram(X, Y), float32, 128 x 128, length 100, dim 8
-                (reference scheme and the fused paths; full 128 x 128 float64 oracle)
+                (reference scheme and the fused paths; full 128 x 128 float64 oracle); the same
+                with RBFKernel / L1RBFKernel (one-pass increments vs the torch route)
       robust    RobustSigPDE(LinearKernel(), 1): .normalization(X) and .gram(X), float64, 256 paths,
                 length 64, dim 16, increments N(0, 9/64), tol 1e-10 (root-find iteration histogram)
     """
@@ -334,6 +335,32 @@ def workloads_leg(args, dev):
         if mode == "ref":
             g[mode]["rel_err_vs_fp32_reference_semantics"] = _rel(Kg, K32ref.astype(np.float64))
     out["gram_128x128_L100_d8_o2_f32"] = g
+
+    # the same Gram with the non-linear static kernels: one-pass increments vs the torch route
+    class _TorchRoute:
+        """Hides the static kernel's one-pass increment hooks (the reference's torch route)."""
+        def __init__(self, k):
+            self.k = k
+
+        def pairwise(self, X, Y):
+            return self.k.pairwise(X, Y)
+
+        def gram(self, X, Y):
+            return self.k.gram(X, Y)
+
+    Xs64, Ys64 = X[:16].double().cpu().numpy(), Y[:16].double().cpu().numpy()
+    for kind, K in (("rbf", sig.kernels.RBFKernel(1.0)), ("l1rbf", sig.kernels.L1RBFKernel(1.0))):
+        k1 = sig.SigPDE(K, 2)
+        k2 = sig.SigPDE(_TorchRoute(K), 2)
+        k1.gram(X, Y)
+        k2.gram(X, Y)
+        torch.cuda.synchronize()
+        Kg, ms1 = _event_ms(lambda: k1.gram(X, Y), 3)
+        _, ms2 = _event_ms(lambda: k2.gram(X, Y), 3)
+        ref = oracle.sig_gram(Xs64, Ys64, 2, kind, 1.0)
+        out[f"{kind}_gram_128x128_L100_d8_o2_f32"] = {
+            "one_pass_increments_ms": ms1, "torch_route_ms": ms2,
+            "rel_err_fp64_16x16": _rel(Kg[:16, :16].double().cpu().numpy(), ref)}
 
     # robust, float64, order 1
     Xr = brownian_paths(256, 64, 16, 9.0 / 64, 4, torch.float64, dev)

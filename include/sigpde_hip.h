@@ -141,6 +141,29 @@ int sigpde_linear_pairwise(int dtype, int precision, const void *X, const void *
                            const void *sx, const void *sy, void *out, void *workspace,
                            size_t workspace_bytes, void *stream);
 
+/*
+ * Increments of the non-linear static kernels in one pass. These replace the chain
+ * RBFKernel / L1RBFKernel .pairwise / .gram (sigpde/torch/static_kernels.py:13-54) followed by
+ * pairwise_inner_product / gram_inner_product (sigpde/torch/kernel.py:13-21):
+ *   out[a][b][i][j] = (((k[i+1][j+1] + k[i][j]) - k[i+1][j]) - k[i][j+1]) / 4^order,
+ *   i < m, j < n, out C-contiguous (A, B, m, n); pairwise calls use B = 1.
+ * sigpde_rbf_increments: k[i][j] = exp(-((-2 ip) + (xx + yy)) * (1 / sigma)) in dtype, where
+ *   ip (a, b, i, j) = <x_i, y_j> sits at a*s[0] + b*s[1] + i*s[2] + j*s[3] (ip_strides, elements;
+ *   points i <= m, j <= n), xx (a, i) at a*xx_stride_a + i, yy (a, b, j) at a*yy_stride_a +
+ *   b*yy_stride_b + j: the reference's RBF formula operation for operation, from the caller's
+ *   GEMM and row sums.
+ * sigpde_l1rbf_increments: k[i][j] = exp(-(sum_k |x_ik - y_jk|) * (1 / sigma)) from the paths
+ *   X (A, m+1, d), Y (B, n+1, d), C-contiguous; pairwise != 0 pairs X_a with Y_a (B = A).
+ */
+int sigpde_rbf_increments(int dtype, const void *ip, int64_t A, int64_t B, int64_t m, int64_t n,
+                          const int64_t *ip_strides, const void *xx, int64_t xx_stride_a,
+                          const void *yy, int64_t yy_stride_a, int64_t yy_stride_b, double sigma,
+                          int order, void *out, void *stream);
+
+int sigpde_l1rbf_increments(int dtype, const void *X, int64_t A, const void *Y, int64_t B,
+                            int64_t m, int64_t n, int64_t d, int pairwise, double sigma, int order,
+                            void *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,10 @@ _SIGNATURES = {
                            _vp, _int, _vp, _i64, _vp, ctypes.c_size_t, _vp],
     "sigpde_linear_pairwise": [_int, _int, _vp, _vp, _i64, _i64, _i64, _i64, _dbl, _int, _vp,
                                _vp, _vp, _vp, ctypes.c_size_t, _vp],
+    "sigpde_rbf_increments": [_int, _vp, _i64, _i64, _i64, _i64, ctypes.POINTER(_i64), _vp, _i64,
+                              _vp, _i64, _i64, _dbl, _int, _vp, _vp],
+    "sigpde_l1rbf_increments": [_int, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _int, _dbl, _int,
+                                _vp, _vp],
 }
 
 
@@ -189,3 +193,31 @@ def linear_pairwise(X: torch.Tensor, Y: torch.Tensor, order: int, kscale: float,
                                   float(kscale), int(order), _ptr(sx), _ptr(sy), _ptr(out),
                                   _ptr(ws), nbytes, _stream(X.device))
     _check(rc, "sigpde_linear_pairwise")
+
+
+def rbf_increments(ip: torch.Tensor, ip_strides, A: int, B: int, m: int, n: int,
+                   xx: torch.Tensor, xx_stride_a: int, yy: torch.Tensor, yy_stride_a: int,
+                   yy_stride_b: int, sigma: float, order: int, out: torch.Tensor) -> None:
+    """RBF increments (A, B, m, n) from the point inner products ip (element (a, b, i, j) at the
+    given element strides) and the row sums of squares xx (a, i), yy (a, b, j)."""
+    _require_gpu(ip, xx, yy, out)
+    assert out.is_contiguous() and out.numel() == A * B * m * n
+    strides = (_i64 * 4)(*[int(v) for v in ip_strides])
+    rc = load().sigpde_rbf_increments(dtype_code(ip.dtype), _ptr(ip), A, B, m, n, strides,
+                                      _ptr(xx), int(xx_stride_a), _ptr(yy), int(yy_stride_a),
+                                      int(yy_stride_b), float(sigma), int(order), _ptr(out),
+                                      _stream(ip.device))
+    _check(rc, "sigpde_rbf_increments")
+
+
+def l1rbf_increments(X: torch.Tensor, Y: torch.Tensor, sigma: float, order: int,
+                     out: torch.Tensor, pairwise: bool) -> None:
+    """L1-RBF increments from the paths: (A, m, n) pairwise (X_a with Y_a) or (A, B, m, n)."""
+    _require_gpu(X, Y, out)
+    assert X.is_contiguous() and Y.is_contiguous() and out.is_contiguous()
+    A, Lx, d = X.shape
+    B, Ly, _ = Y.shape
+    rc = load().sigpde_l1rbf_increments(dtype_code(X.dtype), _ptr(X), A, _ptr(Y), B, Lx - 1,
+                                        Ly - 1, d, int(pairwise), float(sigma), int(order),
+                                        _ptr(out), _stream(X.device))
+    _check(rc, "sigpde_l1rbf_increments")

@@ -34,6 +34,7 @@ SOURCES = {
     # they would be copied out every period)
     "fused_linear.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
     "fused_pairwise.hip": [],
+    "static_increments.hip": [],
 }
 
 

@@ -1,0 +1,212 @@
+// Increments of the non-linear static kernels in one pass (the reference's RBF / L1-RBF point
+// kernels, sigpde/torch/static_kernels.py:13-54, followed by the 4-term second difference of
+// sigpde/torch/kernel.py:13-21):
+//   inc[i][j] = (((k[i+1][j+1] + k[i][j]) - k[i+1][j]) - k[i][j+1]) * 4^-order
+// The reference materialises the point Gram and then about ten full-size temporaries (the
+// distance, its scaling, exp, three slice differences, the division); here a 16 x 16 tile of
+// increments reads its 17 x 17 point values once, keeps k in LDS and writes the increments once.
+//   RBF:    k = exp(-((-2 <x,y>) + (|x|^2 + |y|^2)) * (1/sigma)) with <x,y> the caller's GEMM output
+//           and |x|^2, |y|^2 the caller's row sums: the same float operations in the same order
+//           as the torch formulation (division by the Python scalar sigma is a multiplication by
+//           its reciprocal in the operand type), so the increments equal the torch path's.
+//   L1-RBF: k = exp(-(sum_k |x_k - y_k|) * (1/sigma)), the distance summed here over k in order
+//           (no (A, B, M, N, D) temporary); the torch reduction sums in another order, so these
+//           agree with it to float rounding.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "capi_util.h"
+
+#pragma clang fp contract(off)
+
+namespace sigpde {
+
+constexpr int kIncTile = 16;             // increments per tile side
+constexpr int kPtTile = kIncTile + 1;    // point values per tile side
+constexpr int kDimChunk = 32;            // L1: dimensions staged in LDS at a time
+
+// Tile decomposition of a 1-D grid: blocks enumerate (pair, tile row, tile column).
+struct IncGeom {
+  int64_t pairs;  // A * B (Gram) or A (pairwise)
+  int64_t B;      // y paths (1 for pairwise)
+  int m, n;       // increments per x / y path (M - 1, N - 1)
+  int ti, tj;     // tiles per side
+  __device__ void tile(int64_t blk, int64_t &a, int64_t &b, int &i0, int &j0) const {
+    const int64_t per = (int64_t)ti * tj;
+    const int64_t p = blk / per;
+    const int r = (int)(blk - p * per);
+    a = p / B;
+    b = p - a * B;
+    i0 = (r / tj) * kIncTile;
+    j0 = (r % tj) * kIncTile;
+  }
+};
+
+// The 16 x 16 increments of a tile from the 17 x 17 point values in `kt`.
+template <typename T>
+__device__ __forceinline__ void write_tile(const T (*kt)[kPtTile + 1], const IncGeom &g,
+                                           int64_t a, int64_t b, int i0, int j0, T inv4,
+                                           T *out) {
+  const int t = threadIdx.x;
+  const int li = t / kIncTile, lj = t % kIncTile;
+  const int i = i0 + li, j = j0 + lj;
+  if (i < g.m && j < g.n) {
+    const T d = ((kt[li + 1][lj + 1] + kt[li][lj]) - kt[li + 1][lj]) - kt[li][lj + 1];
+    out[((a * g.B + b) * g.m + i) * (int64_t)g.n + j] = d * inv4;
+  }
+}
+
+// RBF. ip element (a, b, i, j) at a*sa + b*sb + i*si + j*sj (points i < m+1, j < n+1);
+// xx (a, i) at a*xa + i; yy (a, b, j) at a*ya + b*yb + j.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rbf_increments(const T *__restrict__ ip, int64_t sa,
+                                                        int64_t sb, int64_t si, int64_t sj,
+                                                        const T *__restrict__ xx, int64_t xa,
+                                                        const T *__restrict__ yy, int64_t ya,
+                                                        int64_t yb, IncGeom g, T inv_sigma,
+                                                        T inv4, T *__restrict__ out) {
+  __shared__ T kt[kPtTile][kPtTile + 1];
+  int64_t a, b;
+  int i0, j0;
+  g.tile(blockIdx.x, a, b, i0, j0);
+  for (int p = threadIdx.x; p < kPtTile * kPtTile; p += blockDim.x) {
+    const int li = p / kPtTile, lj = p % kPtTile;
+    const int i = min(i0 + li, g.m), j = min(j0 + lj, g.n);  // clamped: unused past the edge
+    const T v = ip[a * sa + b * sb + i * si + j * sj];
+    T dist = (T)-2 * v;
+    dist += xx[a * xa + i] + yy[a * ya + b * yb + j];
+    kt[li][lj] = exp((-dist) * inv_sigma);
+  }
+  __syncthreads();
+  write_tile<T>(kt, g, a, b, i0, j0, inv4, out);
+}
+
+// L1-RBF. X (A, m+1, D), Y (By, n+1, D) contiguous; pairwise: y path = x path (ypair = 1).
+template <typename T>
+__global__ __launch_bounds__(256) void k_l1rbf_increments(const T *__restrict__ X,
+                                                          const T *__restrict__ Y, int D,
+                                                          int ypair, IncGeom g, T inv_sigma,
+                                                          T inv4, T *__restrict__ out) {
+  __shared__ T kt[kPtTile][kPtTile + 1];
+  __shared__ T xs[kPtTile][kDimChunk + 1], ys[kPtTile][kDimChunk + 1];
+  int64_t a, b;
+  int i0, j0;
+  g.tile(blockIdx.x, a, b, i0, j0);
+  const int64_t yp = ypair ? a : b;
+  const T *xr = X + a * (int64_t)(g.m + 1) * D;
+  const T *yr = Y + yp * (int64_t)(g.n + 1) * D;
+  // each thread owns point values p = threadIdx.x and threadIdx.x + 256 (289 per tile)
+  T acc0 = 0, acc1 = 0;
+  const int p0 = threadIdx.x, p1 = threadIdx.x + 256;
+  for (int k0 = 0; k0 < D; k0 += kDimChunk) {
+    const int kc = min(kDimChunk, D - k0);
+    for (int e = threadIdx.x; e < kPtTile * kDimChunk; e += blockDim.x) {
+      const int r = e / kDimChunk, k = e % kDimChunk;
+      const int i = min(i0 + r, g.m), j = min(j0 + r, g.n);
+      xs[r][k] = k < kc ? xr[(int64_t)i * D + k0 + k] : (T)0;
+      ys[r][k] = k < kc ? yr[(int64_t)j * D + k0 + k] : (T)0;
+    }
+    __syncthreads();
+    {
+      const int li = p0 / kPtTile, lj = p0 % kPtTile;
+      for (int k = 0; k < kc; ++k) acc0 += fabs(xs[li][k] - ys[lj][k]);
+    }
+    if (p1 < kPtTile * kPtTile) {
+      const int li = p1 / kPtTile, lj = p1 % kPtTile;
+      for (int k = 0; k < kc; ++k) acc1 += fabs(xs[li][k] - ys[lj][k]);
+    }
+    __syncthreads();
+  }
+  kt[p0 / kPtTile][p0 % kPtTile] = exp((-acc0) * inv_sigma);
+  if (p1 < kPtTile * kPtTile) kt[p1 / kPtTile][p1 % kPtTile] = exp((-acc1) * inv_sigma);
+  __syncthreads();
+  write_tile<T>(kt, g, a, b, i0, j0, inv4, out);
+}
+
+static IncGeom inc_geom(int64_t A, int64_t B, int64_t m, int64_t n) {
+  IncGeom g;
+  g.pairs = A * B;
+  g.B = B;
+  g.m = (int)m;
+  g.n = (int)n;
+  g.ti = (int)((m + kIncTile - 1) / kIncTile);
+  g.tj = (int)((n + kIncTile - 1) / kIncTile);
+  return g;
+}
+
+static int check_inc(int dtype, int64_t A, int64_t B, int64_t m, int64_t n, int order,
+                     double sigma) {
+  if (dtype != SIGPDE_F32 && dtype != SIGPDE_F64)
+    return set_error(SIGPDE_EINVAL, "increments: unknown dtype %d", dtype);
+  if (A < 0 || B < 0 || m < 1 || n < 1)
+    return set_error(SIGPDE_EINVAL, "increments: bad sizes (paths need length >= 2)");
+  if (m > (1 << 24) || n > (1 << 24))
+    return set_error(SIGPDE_EINVAL, "increments: paths longer than 2^24 points");
+  if (order < 0 || order > 12) return set_error(SIGPDE_EINVAL, "increments: dyadic order must be in [0, 12]");
+  if (!(sigma != 0.0)) return set_error(SIGPDE_EINVAL, "increments: sigma must be nonzero");
+  return SIGPDE_OK;
+}
+
+// Reciprocals in the operand type, as torch forms them for a division by a Python scalar.
+template <typename T>
+static T recip(double v) {
+  return (T)1 / (T)v;
+}
+
+}  // namespace sigpde
+
+using namespace sigpde;
+
+extern "C" int sigpde_rbf_increments(int dtype, const void *ip, int64_t A, int64_t B, int64_t m,
+                                     int64_t n, const int64_t *ip_strides, const void *xx,
+                                     int64_t xx_stride_a, const void *yy, int64_t yy_stride_a,
+                                     int64_t yy_stride_b, double sigma, int order, void *out,
+                                     void *stream) {
+  clear_error();
+  if (int rc = check_inc(dtype, A, B, m, n, order, sigma)) return rc;
+  if (A == 0 || B == 0) return SIGPDE_OK;
+  if (!ip || !ip_strides || !xx || !yy || !out)
+    return set_error(SIGPDE_EINVAL, "rbf_increments: null pointer");
+  const IncGeom g = inc_geom(A, B, m, n);
+  const int64_t blocks = g.pairs * g.ti * g.tj;
+  if (blocks > INT32_MAX) return set_error(SIGPDE_EINVAL, "rbf_increments: too many tiles");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t *s = ip_strides;
+  if (dtype == SIGPDE_F32) {
+    k_rbf_increments<float><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+        (const float *)ip, s[0], s[1], s[2], s[3], (const float *)xx, xx_stride_a,
+        (const float *)yy, yy_stride_a, yy_stride_b, g, recip<float>(sigma),
+        recip<float>((double)(1ll << (2 * order))), (float *)out);
+  } else {
+    k_rbf_increments<double><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+        (const double *)ip, s[0], s[1], s[2], s[3], (const double *)xx, xx_stride_a,
+        (const double *)yy, yy_stride_a, yy_stride_b, g, recip<double>(sigma),
+        recip<double>((double)(1ll << (2 * order))), (double *)out);
+  }
+  return hip_status(hipGetLastError(), "rbf_increments launch");
+}
+
+extern "C" int sigpde_l1rbf_increments(int dtype, const void *X, int64_t A, const void *Y,
+                                       int64_t B, int64_t m, int64_t n, int64_t d, int pairwise,
+                                       double sigma, int order, void *out, void *stream) {
+  clear_error();
+  if (int rc = check_inc(dtype, A, pairwise ? 1 : B, m, n, order, sigma)) return rc;
+  if (A == 0 || (!pairwise && B == 0)) return SIGPDE_OK;
+  if (!X || !Y || !out) return set_error(SIGPDE_EINVAL, "l1rbf_increments: null pointer");
+  if (d < 1 || d > (1 << 20)) return set_error(SIGPDE_EINVAL, "l1rbf_increments: bad dim");
+  const IncGeom g = inc_geom(A, pairwise ? 1 : B, m, n);
+  const int64_t blocks = g.pairs * g.ti * g.tj;
+  if (blocks > INT32_MAX) return set_error(SIGPDE_EINVAL, "l1rbf_increments: too many tiles");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == SIGPDE_F32) {
+    k_l1rbf_increments<float><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+        (const float *)X, (const float *)Y, (int)d, pairwise, g, recip<float>(sigma),
+        recip<float>((double)(1ll << (2 * order))), (float *)out);
+  } else {
+    k_l1rbf_increments<double><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+        (const double *)X, (const double *)Y, (int)d, pairwise, g, recip<double>(sigma),
+        recip<double>((double)(1ll << (2 * order))), (double *)out);
+  }
+  return hip_status(hipGetLastError(), "l1rbf_increments launch");
+}

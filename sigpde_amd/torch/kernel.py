@@ -38,14 +38,24 @@ from .. import _native
 
 
 def pairwise_inner_product(x, y, static_kernel, dyadic_order):
-    """Increments (B, Lx-1, Ly-1); sigpde/torch/kernel.py:13-16."""
+    """Increments (B, Lx-1, Ly-1); sigpde/torch/kernel.py:13-16. Static kernels with a one-pass
+    `pairwise_increments` (RBF, L1-RBF) use it."""
+    hook = getattr(static_kernel, "pairwise_increments", None)
+    inc = hook(x, y, dyadic_order) if hook is not None else None
+    if inc is not None:
+        return inc
     ip = static_kernel.pairwise(x, y)
     d = ip[:, 1:, 1:] + ip[:, :-1, :-1] - ip[:, 1:, :-1] - ip[:, :-1, 1:]
     return d / float(2 ** (2 * dyadic_order))
 
 
 def gram_inner_product(x, y, static_kernel, dyadic_order):
-    """Increments (Bx, By, Lx-1, Ly-1); sigpde/torch/kernel.py:18-21."""
+    """Increments (Bx, By, Lx-1, Ly-1); sigpde/torch/kernel.py:18-21. Static kernels with a one-pass
+    `gram_increments` (RBF, L1-RBF) use it."""
+    hook = getattr(static_kernel, "gram_increments", None)
+    inc = hook(x, y, dyadic_order) if hook is not None else None
+    if inc is not None:
+        return inc
     ip = static_kernel.gram(x, y)
     d = ip[:, :, 1:, 1:] + ip[:, :, :-1, :-1] - ip[:, :, 1:, :-1] - ip[:, :, :-1, 1:]
     return d / float(2 ** (2 * dyadic_order))

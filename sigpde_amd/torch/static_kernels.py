@@ -9,11 +9,18 @@ static kernels. The formulas are the reference's (they define the API's numbers)
   RBF     exp(-(|x|^2 + |y|^2 - 2 <x, y>) / sigma)      (sigma not squared, no 1/2)
   L1-RBF  exp(-sum_k |x_k - y_k| / sigma)
 LinearKernel additionally advertises `fused_linear_scale`, which lets SigPDE route it to the
-fused increment-Gram kernels instead of materialising the point Gram.
+fused increment-Gram kernels instead of materialising the point Gram. RBFKernel and L1RBFKernel
+advertise `pairwise_increments` / `gram_increments`: the increments of the API's second difference
+(sigpde/torch/kernel.py:13-21) computed in one HIP pass from the point inner products (RBF: the same
+float operations as the torch formulas below, so the same increments) or from the paths (L1-RBF:
+without the (A, B, M, N, D) difference tensor). A subclass that overrides pairwise / gram gets the
+generic torch route instead.
 """
 from __future__ import annotations
 
 import torch
+
+from .. import _native
 
 
 class LinearKernel:
@@ -58,6 +65,36 @@ class RBFKernel:
         dist += xx + yy
         return torch.exp(-dist / self.sigma)
 
+    def pairwise_increments(self, X: torch.Tensor, Y: torch.Tensor, dyadic_order: int):
+        """(A, M-1, N-1) second differences of pairwise(X, Y) / 4^order in one pass, or None when
+        a subclass overrides pairwise."""
+        if type(self).pairwise is not RBFKernel.pairwise:
+            return None
+        A, M, _ = X.shape
+        N = Y.shape[1]
+        xx = (X ** 2).sum(dim=2)
+        yy = (Y ** 2).sum(dim=2)
+        ip = torch.bmm(X, Y.transpose(1, 2).contiguous())
+        out = torch.empty((A, M - 1, N - 1), dtype=X.dtype, device=X.device)
+        _native.rbf_increments(ip, (M * N, 0, N, 1), A, 1, M - 1, N - 1, xx, M, yy, N, 0,
+                               self.sigma, dyadic_order, out)
+        return out
+
+    def gram_increments(self, X: torch.Tensor, Y: torch.Tensor, dyadic_order: int):
+        """(A, B, M-1, N-1) second differences of gram(X, Y) / 4^order in one pass, or None when
+        a subclass overrides gram."""
+        if type(self).gram is not RBFKernel.gram:
+            return None
+        A, M, D = X.shape
+        B, N, _ = Y.shape
+        xx = (X ** 2).sum(dim=2)
+        yy = (Y ** 2).sum(dim=2)
+        ip = torch.mm(X.reshape(A * M, D), Y.reshape(B * N, D).t())  # element (a, i, b, j)
+        out = torch.empty((A, B, M - 1, N - 1), dtype=X.dtype, device=X.device)
+        _native.rbf_increments(ip, (M * B * N, N, B * N, 1), A, B, M - 1, N - 1, xx, M, yy, 0,
+                               N, self.sigma, dyadic_order, out)
+        return out
+
 
 class L1RBFKernel:
     def __init__(self, sigma: float = 1.0):
@@ -70,3 +107,27 @@ class L1RBFKernel:
     def gram(self, X: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
         dist = (X[:, None, :, None, :] - Y[None, :, None, :, :]).abs().sum(dim=-1)
         return torch.exp(-dist / self.sigma)
+
+    def pairwise_increments(self, X: torch.Tensor, Y: torch.Tensor, dyadic_order: int):
+        """(A, M-1, N-1) second differences of pairwise(X, Y) / 4^order from the paths, or None
+        when a subclass overrides pairwise."""
+        if type(self).pairwise is not L1RBFKernel.pairwise:
+            return None
+        A, M, _ = X.shape
+        N = Y.shape[1]
+        out = torch.empty((A, M - 1, N - 1), dtype=X.dtype, device=X.device)
+        _native.l1rbf_increments(X.contiguous(), Y.contiguous(), self.sigma, dyadic_order, out,
+                                 pairwise=True)
+        return out
+
+    def gram_increments(self, X: torch.Tensor, Y: torch.Tensor, dyadic_order: int):
+        """(A, B, M-1, N-1) second differences of gram(X, Y) / 4^order from the paths, or None
+        when a subclass overrides gram."""
+        if type(self).gram is not L1RBFKernel.gram:
+            return None
+        A, M, _ = X.shape
+        B, N, _ = Y.shape
+        out = torch.empty((A, B, M - 1, N - 1), dtype=X.dtype, device=X.device)
+        _native.l1rbf_increments(X.contiguous(), Y.contiguous(), self.sigma, dyadic_order, out,
+                                 pairwise=False)
+        return out
