@@ -164,6 +164,14 @@ int sigpde_l1rbf_increments(int dtype, const void *X, int64_t A, const void *Y, 
                             int64_t m, int64_t n, int64_t d, int pairwise, double sigma, int order,
                             void *out, void *stream);
 
+/* The same for the linear static kernel, k[i][j] = kscale * ip (a, b, i, j) with ip laid out as
+ * for sigpde_rbf_increments: the reference scheme's increments (LinearKernel.gram / .pairwise then
+ * the second difference, sigpde/torch/static_kernels.py:3-11, sigpde/torch/kernel.py:13-21) in one
+ * pass, operation for operation. */
+int sigpde_linear_increments(int dtype, const void *ip, int64_t A, int64_t B, int64_t m, int64_t n,
+                             const int64_t *ip_strides, double kscale, int order, void *out,
+                             void *stream);
+
 #ifdef __cplusplus
 }
 #endif

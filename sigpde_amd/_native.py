@@ -45,6 +45,8 @@ _SIGNATURES = {
                               _vp, _i64, _i64, _dbl, _int, _vp, _vp],
     "sigpde_l1rbf_increments": [_int, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _int, _dbl, _int,
                                 _vp, _vp],
+    "sigpde_linear_increments": [_int, _vp, _i64, _i64, _i64, _i64, ctypes.POINTER(_i64), _dbl,
+                                 _int, _vp, _vp],
 }
 
 
@@ -221,3 +223,15 @@ def l1rbf_increments(X: torch.Tensor, Y: torch.Tensor, sigma: float, order: int,
                                         Ly - 1, d, int(pairwise), float(sigma), int(order),
                                         _ptr(out), _stream(X.device))
     _check(rc, "sigpde_l1rbf_increments")
+
+
+def linear_increments(ip: torch.Tensor, ip_strides, A: int, B: int, m: int, n: int,
+                      kscale: float, order: int, out: torch.Tensor) -> None:
+    """Linear-kernel increments (A, B, m, n) of kscale * ip (element (a, b, i, j) at the given
+    element strides): the reference scheme's second difference in one pass."""
+    _require_gpu(ip, out)
+    assert out.is_contiguous() and out.numel() == A * B * m * n
+    strides = (_i64 * 4)(*[int(v) for v in ip_strides])
+    rc = load().sigpde_linear_increments(dtype_code(ip.dtype), _ptr(ip), A, B, m, n, strides,
+                                         float(kscale), int(order), _ptr(out), _stream(ip.device))
+    _check(rc, "sigpde_linear_increments")

@@ -1,12 +1,14 @@
-// Increments of the non-linear static kernels in one pass (the reference's RBF / L1-RBF point
-// kernels, sigpde/torch/static_kernels.py:13-54, followed by the 4-term second difference of
+// Increments of the static kernels in one pass (the reference's point kernels,
+// sigpde/torch/static_kernels.py:3-54, followed by the 4-term second difference of
 // sigpde/torch/kernel.py:13-21):
 //   inc[i][j] = (((k[i+1][j+1] + k[i][j]) - k[i+1][j]) - k[i][j+1]) * 4^-order
 // The reference materialises the point Gram and then about ten full-size temporaries (the
 // distance, its scaling, exp, three slice differences, the division); here a 16 x 16 tile of
 // increments reads its 17 x 17 point values once, keeps k in LDS and writes the increments once.
-//   RBF:    k = exp(-((-2 <x,y>) + (|x|^2 + |y|^2)) * (1/sigma)) with <x,y> the caller's GEMM output
-//           and |x|^2, |y|^2 the caller's row sums: the same float operations in the same order
+//   linear: k = kscale * <x,y> with <x,y> the caller's GEMM output (the reference scheme's
+//           increments for precision "ref", bit-equal to the torch formulation);
+//   RBF:    k = exp(-((-2 <x,y>) + (|x|^2 + |y|^2)) * (1/sigma)) with <x,y> the caller's GEMM
+//           output and |x|^2, |y|^2 the caller's row sums: the same float operations in the order
 //           as the torch formulation (division by the Python scalar sigma is a multiplication by
 //           its reciprocal in the operand type), so the increments equal the torch path's.
 //   L1-RBF: k = exp(-(sum_k |x_k - y_k|) * (1/sigma)), the distance summed here over k in order
@@ -57,15 +59,16 @@ __device__ __forceinline__ void write_tile(const T (*kt)[kPtTile + 1], const Inc
   }
 }
 
-// RBF. ip element (a, b, i, j) at a*sa + b*sb + i*si + j*sj (points i < m+1, j < n+1);
-// xx (a, i) at a*xa + i; yy (a, b, j) at a*ya + b*yb + j.
-template <typename T>
-__global__ __launch_bounds__(256) void k_rbf_increments(const T *__restrict__ ip, int64_t sa,
-                                                        int64_t sb, int64_t si, int64_t sj,
-                                                        const T *__restrict__ xx, int64_t xa,
-                                                        const T *__restrict__ yy, int64_t ya,
-                                                        int64_t yb, IncGeom g, T inv_sigma,
-                                                        T inv4, T *__restrict__ out) {
+// From point inner products. ip element (a, b, i, j) at a*sa + b*sb + i*si + j*sj (points
+// i < m+1, j < n+1). RBF (RBF = true): xx (a, i) at a*xa + i, yy (a, b, j) at a*ya + b*yb + j,
+// c = 1/sigma; linear: c = kscale (xx, yy unused).
+template <typename T, bool RBF>
+__global__ __launch_bounds__(256) void k_ip_increments(const T *__restrict__ ip, int64_t sa,
+                                                       int64_t sb, int64_t si, int64_t sj,
+                                                       const T *__restrict__ xx, int64_t xa,
+                                                       const T *__restrict__ yy, int64_t ya,
+                                                       int64_t yb, IncGeom g, T c, T inv4,
+                                                       T *__restrict__ out) {
   __shared__ T kt[kPtTile][kPtTile + 1];
   int64_t a, b;
   int i0, j0;
@@ -74,9 +77,13 @@ __global__ __launch_bounds__(256) void k_rbf_increments(const T *__restrict__ ip
     const int li = p / kPtTile, lj = p % kPtTile;
     const int i = min(i0 + li, g.m), j = min(j0 + lj, g.n);  // clamped: unused past the edge
     const T v = ip[a * sa + b * sb + i * si + j * sj];
-    T dist = (T)-2 * v;
-    dist += xx[a * xa + i] + yy[a * ya + b * yb + j];
-    kt[li][lj] = exp((-dist) * inv_sigma);
+    if constexpr (RBF) {
+      T dist = (T)-2 * v;
+      dist += xx[a * xa + i] + yy[a * ya + b * yb + j];
+      kt[li][lj] = exp((-dist) * c);
+    } else {
+      kt[li][lj] = c * v;
+    }
   }
   __syncthreads();
   write_tile<T>(kt, g, a, b, i0, j0, inv4, out);
@@ -174,17 +181,41 @@ extern "C" int sigpde_rbf_increments(int dtype, const void *ip, int64_t A, int64
   hipStream_t st = (hipStream_t)stream;
   const int64_t *s = ip_strides;
   if (dtype == SIGPDE_F32) {
-    k_rbf_increments<float><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+    k_ip_increments<float, true><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
         (const float *)ip, s[0], s[1], s[2], s[3], (const float *)xx, xx_stride_a,
         (const float *)yy, yy_stride_a, yy_stride_b, g, recip<float>(sigma),
         recip<float>((double)(1ll << (2 * order))), (float *)out);
   } else {
-    k_rbf_increments<double><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+    k_ip_increments<double, true><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
         (const double *)ip, s[0], s[1], s[2], s[3], (const double *)xx, xx_stride_a,
         (const double *)yy, yy_stride_a, yy_stride_b, g, recip<double>(sigma),
         recip<double>((double)(1ll << (2 * order))), (double *)out);
   }
   return hip_status(hipGetLastError(), "rbf_increments launch");
+}
+
+extern "C" int sigpde_linear_increments(int dtype, const void *ip, int64_t A, int64_t B,
+                                        int64_t m, int64_t n, const int64_t *ip_strides,
+                                        double kscale, int order, void *out, void *stream) {
+  clear_error();
+  if (int rc = check_inc(dtype, A, B, m, n, order, 1.0)) return rc;
+  if (A == 0 || B == 0) return SIGPDE_OK;
+  if (!ip || !ip_strides || !out) return set_error(SIGPDE_EINVAL, "linear_increments: null pointer");
+  const IncGeom g = inc_geom(A, B, m, n);
+  const int64_t blocks = g.pairs * g.ti * g.tj;
+  if (blocks > INT32_MAX) return set_error(SIGPDE_EINVAL, "linear_increments: too many tiles");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t *s = ip_strides;
+  if (dtype == SIGPDE_F32) {
+    k_ip_increments<float, false><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+        (const float *)ip, s[0], s[1], s[2], s[3], nullptr, 0, nullptr, 0, 0, g, (float)kscale,
+        recip<float>((double)(1ll << (2 * order))), (float *)out);
+  } else {
+    k_ip_increments<double, false><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+        (const double *)ip, s[0], s[1], s[2], s[3], nullptr, 0, nullptr, 0, 0, g, kscale,
+        recip<double>((double)(1ll << (2 * order))), (double *)out);
+  }
+  return hip_status(hipGetLastError(), "linear_increments launch");
 }
 
 extern "C" int sigpde_l1rbf_increments(int dtype, const void *X, int64_t A, const void *Y,

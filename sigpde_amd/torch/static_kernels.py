@@ -9,11 +9,11 @@ static kernels. The formulas are the reference's (they define the API's numbers)
   RBF     exp(-(|x|^2 + |y|^2 - 2 <x, y>) / sigma)      (sigma not squared, no 1/2)
   L1-RBF  exp(-sum_k |x_k - y_k| / sigma)
 LinearKernel additionally advertises `fused_linear_scale`, which lets SigPDE route it to the
-fused increment-Gram kernels instead of materialising the point Gram. RBFKernel and L1RBFKernel
-advertise `pairwise_increments` / `gram_increments`: the increments of the API's second difference
-(sigpde/torch/kernel.py:13-21) computed in one HIP pass from the point inner products (RBF: the same
-float operations as the torch formulas below, so the same increments) or from the paths (L1-RBF:
-without the (A, B, M, N, D) difference tensor). A subclass that overrides pairwise / gram gets the
+fused increment-Gram kernels instead of materialising the point Gram. All three advertise
+`pairwise_increments` / `gram_increments`: the increments of the API's second difference
+(sigpde/torch/kernel.py:13-21) computed in one HIP pass from the point inner products (linear and
+RBF: the same float operations as the torch formulas below, so the same increments) or from the
+paths (L1-RBF: without the (A, B, M, N, D) difference tensor). A subclass that overrides pairwise / gram gets the
 generic torch route instead.
 """
 from __future__ import annotations
@@ -40,6 +40,32 @@ class LinearKernel:
         # (A*M, D) x (D, B*N) as one GEMM, then (A, M, B, N) -> (A, B, M, N)
         ip = torch.mm(X.reshape(A * M, D), Y.reshape(B * N, D).t())
         return self.scale * ip.view(A, M, B, N).permute(0, 2, 1, 3)
+
+    def pairwise_increments(self, X: torch.Tensor, Y: torch.Tensor, dyadic_order: int):
+        """(A, M-1, N-1) second differences of pairwise(X, Y) / 4^order in one pass (the same
+        float operations), or None when a subclass overrides pairwise."""
+        if type(self).pairwise is not LinearKernel.pairwise:
+            return None
+        A, M, _ = X.shape
+        N = Y.shape[1]
+        ip = torch.bmm(X, Y.transpose(1, 2).contiguous())
+        out = torch.empty((A, M - 1, N - 1), dtype=X.dtype, device=X.device)
+        _native.linear_increments(ip, (M * N, 0, N, 1), A, 1, M - 1, N - 1, self.scale,
+                                  dyadic_order, out)
+        return out
+
+    def gram_increments(self, X: torch.Tensor, Y: torch.Tensor, dyadic_order: int):
+        """(A, B, M-1, N-1) second differences of gram(X, Y) / 4^order in one pass (the same
+        float operations), or None when a subclass overrides gram."""
+        if type(self).gram is not LinearKernel.gram:
+            return None
+        A, M, D = X.shape
+        B, N, _ = Y.shape
+        ip = torch.mm(X.reshape(A * M, D), Y.reshape(B * N, D).t())  # element (a, i, b, j)
+        out = torch.empty((A, B, M - 1, N - 1), dtype=X.dtype, device=X.device)
+        _native.linear_increments(ip, (M * B * N, N, B * N, 1), A, B, M - 1, N - 1, self.scale,
+                                  dyadic_order, out)
+        return out
 
 
 class RBFKernel:

@@ -1,8 +1,9 @@
-"""GPU checks of the one-pass increments of the non-linear static kernels
+"""GPU checks of the one-pass increments of the static kernels
 (sigpde_amd/csrc/static_increments.hip) against the torch formulation of the reference
 (point Gram by the static kernel's own gram / pairwise, then the 4-term second difference,
 sigpde/torch/kernel.py:13-21):
-  RBF: the same float operations in the same order, so the increments must be equal bit for bit;
+  linear and RBF: the same float operations in the same order, so the increments must be equal
+    bit for bit;
   L1-RBF: the distance is summed in another order than torch's reduction, so they agree to the
     operand type's rounding (relative to the point kernel values, which are at most 1);
 and end to end through SigPDE against the float64 oracle, plus the fallback to the torch route
@@ -33,6 +34,16 @@ def torch_gram_inc(k, x, y, order):
 
 def paths(B, L, d, seed, dtype):
     return torch.from_numpy(brownian_np(B, L, d, 1.0 / L, seed)).to(DEV, dtype)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("order", [0, 2])
+def test_linear_increments_equal_torch_formula(dtype, order):
+    k = sig.kernels.LinearKernel(0.7)
+    x, y = paths(5, 37, 6, 1, dtype), paths(4, 21, 6, 2, dtype)
+    assert torch.equal(k.gram_increments(x, y, order), torch_gram_inc(k, x, y, order))
+    y2 = paths(5, 21, 6, 3, dtype)
+    assert torch.equal(k.pairwise_increments(x, y2, order), torch_pairwise_inc(k, x, y2, order))
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
