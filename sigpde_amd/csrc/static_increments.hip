@@ -3,8 +3,10 @@
 // sigpde/torch/kernel.py:13-21):
 //   inc[i][j] = (((k[i+1][j+1] + k[i][j]) - k[i+1][j]) - k[i][j+1]) * 4^-order
 // The reference materialises the point Gram and then about ten full-size temporaries (the
-// distance, its scaling, exp, three slice differences, the division); here a 16 x 16 tile of
-// increments reads its 17 x 17 point values once, keeps k in LDS and writes the increments once.
+// distance, its scaling, exp, three slice differences, the division); here a tile of increments
+// (32 x 128 from inner products: rows of 129 contiguous reads and 128 contiguous writes; 16 x 16
+// for L1-RBF, whose cost is the distance sums) reads its point values once, keeps k in LDS and
+// writes the increments once.
 //   linear: k = kscale * <x,y> with <x,y> the caller's GEMM output (the reference scheme's
 //           increments for precision "ref", bit-equal to the torch formulation);
 //   RBF:    k = exp(-((-2 <x,y>) + (|x|^2 + |y|^2)) * (1/sigma)) with <x,y> the caller's GEMM
@@ -24,15 +26,16 @@
 
 namespace sigpde {
 
-constexpr int kIncTile = 16;             // increments per tile side
-constexpr int kPtTile = kIncTile + 1;    // point values per tile side
-constexpr int kDimChunk = 32;            // L1: dimensions staged in LDS at a time
+constexpr int kIpRows = 32, kIpCols = 128;  // inner-product route: increments per tile
+constexpr int kL1Tile = 16;                  // L1-RBF: increments per tile side
+constexpr int kDimChunk = 32;                // L1: dimensions staged in LDS at a time
 
 // Tile decomposition of a 1-D grid: blocks enumerate (pair, tile row, tile column).
 struct IncGeom {
   int64_t pairs;  // A * B (Gram) or A (pairwise)
   int64_t B;      // y paths (1 for pairwise)
   int m, n;       // increments per x / y path (M - 1, N - 1)
+  int tr, tc;     // increments per tile (rows, columns)
   int ti, tj;     // tiles per side
   __device__ void tile(int64_t blk, int64_t &a, int64_t &b, int &i0, int &j0) const {
     const int64_t per = (int64_t)ti * tj;
@@ -40,21 +43,24 @@ struct IncGeom {
     const int r = (int)(blk - p * per);
     a = p / B;
     b = p - a * B;
-    i0 = (r / tj) * kIncTile;
-    j0 = (r % tj) * kIncTile;
+    i0 = (r / tj) * tr;
+    j0 = (r % tj) * tc;
   }
 };
 
-// The 16 x 16 increments of a tile from the 17 x 17 point values in `kt`.
-template <typename T>
-__device__ __forceinline__ void write_tile(const T (*kt)[kPtTile + 1], const IncGeom &g,
-                                           int64_t a, int64_t b, int i0, int j0, T inv4,
-                                           T *out) {
-  const int t = threadIdx.x;
-  const int li = t / kIncTile, lj = t % kIncTile;
-  const int i = i0 + li, j = j0 + lj;
-  if (i < g.m && j < g.n) {
-    const T d = ((kt[li + 1][lj + 1] + kt[li][lj]) - kt[li + 1][lj]) - kt[li][lj + 1];
+// The TR x TC increments of a tile from the (TR + 1) x (TC + 1) point values in `kt` (row
+// stride KS), each thread taking columns t % TC of rows t / TC, t / TC + 256 / TC, ...
+template <typename T, int TR, int TC, int KS>
+__device__ __forceinline__ void write_tile(const T *kt, const IncGeom &g, int64_t a, int64_t b,
+                                           int i0, int j0, T inv4, T *out) {
+  const int lj = threadIdx.x % TC;
+  const int j = j0 + lj;
+  if (j >= g.n) return;
+  for (int li = threadIdx.x / TC; li < TR; li += 256 / TC) {
+    const int i = i0 + li;
+    if (i >= g.m) break;
+    const T *k0 = kt + li * KS + lj;
+    const T d = ((k0[KS + 1] + k0[0]) - k0[KS]) - k0[1];
     out[((a * g.B + b) * g.m + i) * (int64_t)g.n + j] = d * inv4;
   }
 }
@@ -69,24 +75,25 @@ __global__ __launch_bounds__(256) void k_ip_increments(const T *__restrict__ ip,
                                                        const T *__restrict__ yy, int64_t ya,
                                                        int64_t yb, IncGeom g, T c, T inv4,
                                                        T *__restrict__ out) {
-  __shared__ T kt[kPtTile][kPtTile + 1];
+  constexpr int KR = kIpRows + 1, KC = kIpCols + 1;
+  __shared__ T kt[KR * KC];
   int64_t a, b;
   int i0, j0;
   g.tile(blockIdx.x, a, b, i0, j0);
-  for (int p = threadIdx.x; p < kPtTile * kPtTile; p += blockDim.x) {
-    const int li = p / kPtTile, lj = p % kPtTile;
+  for (int p = threadIdx.x; p < KR * KC; p += blockDim.x) {
+    const int li = p / KC, lj = p % KC;
     const int i = min(i0 + li, g.m), j = min(j0 + lj, g.n);  // clamped: unused past the edge
     const T v = ip[a * sa + b * sb + i * si + j * sj];
     if constexpr (RBF) {
       T dist = (T)-2 * v;
       dist += xx[a * xa + i] + yy[a * ya + b * yb + j];
-      kt[li][lj] = exp((-dist) * c);
+      kt[p] = exp((-dist) * c);
     } else {
-      kt[li][lj] = c * v;
+      kt[p] = c * v;
     }
   }
   __syncthreads();
-  write_tile<T>(kt, g, a, b, i0, j0, inv4, out);
+  write_tile<T, kIpRows, kIpCols, KC>(kt, g, a, b, i0, j0, inv4, out);
 }
 
 // L1-RBF. X (A, m+1, D), Y (By, n+1, D) contiguous; pairwise: y path = x path (ypair = 1).
@@ -95,8 +102,9 @@ __global__ __launch_bounds__(256) void k_l1rbf_increments(const T *__restrict__ 
                                                           const T *__restrict__ Y, int D,
                                                           int ypair, IncGeom g, T inv_sigma,
                                                           T inv4, T *__restrict__ out) {
-  __shared__ T kt[kPtTile][kPtTile + 1];
-  __shared__ T xs[kPtTile][kDimChunk + 1], ys[kPtTile][kDimChunk + 1];
+  constexpr int kPt = kL1Tile + 1;  // point values per tile side
+  __shared__ T kt[kPt * kPt];
+  __shared__ T xs[kPt][kDimChunk + 1], ys[kPt][kDimChunk + 1];
   int64_t a, b;
   int i0, j0;
   g.tile(blockIdx.x, a, b, i0, j0);
@@ -108,7 +116,7 @@ __global__ __launch_bounds__(256) void k_l1rbf_increments(const T *__restrict__ 
   const int p0 = threadIdx.x, p1 = threadIdx.x + 256;
   for (int k0 = 0; k0 < D; k0 += kDimChunk) {
     const int kc = min(kDimChunk, D - k0);
-    for (int e = threadIdx.x; e < kPtTile * kDimChunk; e += blockDim.x) {
+    for (int e = threadIdx.x; e < kPt * kDimChunk; e += blockDim.x) {
       const int r = e / kDimChunk, k = e % kDimChunk;
       const int i = min(i0 + r, g.m), j = min(j0 + r, g.n);
       xs[r][k] = k < kc ? xr[(int64_t)i * D + k0 + k] : (T)0;
@@ -116,29 +124,31 @@ __global__ __launch_bounds__(256) void k_l1rbf_increments(const T *__restrict__ 
     }
     __syncthreads();
     {
-      const int li = p0 / kPtTile, lj = p0 % kPtTile;
+      const int li = p0 / kPt, lj = p0 % kPt;
       for (int k = 0; k < kc; ++k) acc0 += fabs(xs[li][k] - ys[lj][k]);
     }
-    if (p1 < kPtTile * kPtTile) {
-      const int li = p1 / kPtTile, lj = p1 % kPtTile;
+    if (p1 < kPt * kPt) {
+      const int li = p1 / kPt, lj = p1 % kPt;
       for (int k = 0; k < kc; ++k) acc1 += fabs(xs[li][k] - ys[lj][k]);
     }
     __syncthreads();
   }
-  kt[p0 / kPtTile][p0 % kPtTile] = exp((-acc0) * inv_sigma);
-  if (p1 < kPtTile * kPtTile) kt[p1 / kPtTile][p1 % kPtTile] = exp((-acc1) * inv_sigma);
+  kt[p0] = exp((-acc0) * inv_sigma);
+  if (p1 < kPt * kPt) kt[p1] = exp((-acc1) * inv_sigma);
   __syncthreads();
-  write_tile<T>(kt, g, a, b, i0, j0, inv4, out);
+  write_tile<T, kL1Tile, kL1Tile, kPt>(kt, g, a, b, i0, j0, inv4, out);
 }
 
-static IncGeom inc_geom(int64_t A, int64_t B, int64_t m, int64_t n) {
+static IncGeom inc_geom(int64_t A, int64_t B, int64_t m, int64_t n, int tr, int tc) {
   IncGeom g;
   g.pairs = A * B;
   g.B = B;
   g.m = (int)m;
   g.n = (int)n;
-  g.ti = (int)((m + kIncTile - 1) / kIncTile);
-  g.tj = (int)((n + kIncTile - 1) / kIncTile);
+  g.tr = tr;
+  g.tc = tc;
+  g.ti = (int)((m + tr - 1) / tr);
+  g.tj = (int)((n + tc - 1) / tc);
   return g;
 }
 
@@ -175,7 +185,7 @@ extern "C" int sigpde_rbf_increments(int dtype, const void *ip, int64_t A, int64
   if (A == 0 || B == 0) return SIGPDE_OK;
   if (!ip || !ip_strides || !xx || !yy || !out)
     return set_error(SIGPDE_EINVAL, "rbf_increments: null pointer");
-  const IncGeom g = inc_geom(A, B, m, n);
+  const IncGeom g = inc_geom(A, B, m, n, kIpRows, kIpCols);
   const int64_t blocks = g.pairs * g.ti * g.tj;
   if (blocks > INT32_MAX) return set_error(SIGPDE_EINVAL, "rbf_increments: too many tiles");
   hipStream_t st = (hipStream_t)stream;
@@ -201,7 +211,7 @@ extern "C" int sigpde_linear_increments(int dtype, const void *ip, int64_t A, in
   if (int rc = check_inc(dtype, A, B, m, n, order, 1.0)) return rc;
   if (A == 0 || B == 0) return SIGPDE_OK;
   if (!ip || !ip_strides || !out) return set_error(SIGPDE_EINVAL, "linear_increments: null pointer");
-  const IncGeom g = inc_geom(A, B, m, n);
+  const IncGeom g = inc_geom(A, B, m, n, kIpRows, kIpCols);
   const int64_t blocks = g.pairs * g.ti * g.tj;
   if (blocks > INT32_MAX) return set_error(SIGPDE_EINVAL, "linear_increments: too many tiles");
   hipStream_t st = (hipStream_t)stream;
@@ -226,7 +236,7 @@ extern "C" int sigpde_l1rbf_increments(int dtype, const void *X, int64_t A, cons
   if (A == 0 || (!pairwise && B == 0)) return SIGPDE_OK;
   if (!X || !Y || !out) return set_error(SIGPDE_EINVAL, "l1rbf_increments: null pointer");
   if (d < 1 || d > (1 << 20)) return set_error(SIGPDE_EINVAL, "l1rbf_increments: bad dim");
-  const IncGeom g = inc_geom(A, pairwise ? 1 : B, m, n);
+  const IncGeom g = inc_geom(A, pairwise ? 1 : B, m, n, kL1Tile, kL1Tile);
   const int64_t blocks = g.pairs * g.ti * g.tj;
   if (blocks > INT32_MAX) return set_error(SIGPDE_EINVAL, "l1rbf_increments: too many tiles");
   hipStream_t st = (hipStream_t)stream;
