@@ -58,7 +58,7 @@ FLOPS_PER_CELL = 10  # pde_step as written
 # HBM bytes per launch of the fused fp16x2 Gram at the default workload (1024 x 1024 paths, length
 # 256, dim 64, order 0), from rocprofv3 FETCH_SIZE + WRITE_SIZE passes scaled by a 256 MiB copy
 # (profiles/gram_fp16x2_1024x1024_L256_d64_o0_traffic.md); other workloads report null.
-MEASURED_TRAFFIC = {("fp16x2", 1024, 256, 64, 0): 8.443e9}
+MEASURED_TRAFFIC = {("fp16x2", 1024, 256, 64, 0): 7.718e9}
 
 
 def parse():
