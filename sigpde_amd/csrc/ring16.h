@@ -120,8 +120,9 @@ struct Ring16Writer {
 // Group g serves the pair whose increments (m x n, row stride ld) start at inc[g] (null: an idle
 // slot, served zeros). Lane (c, h) loads element e of pair g = ring row 16g + 4h + e (the lane
 // 4h + e of that group: refined rows R (16 band + 4h + e) ..) at chunk column c: four 64-byte
-// row segments per load. Chunk k is loaded two
-// periods before its use and scaled when stored.
+// row segments per load. Chunk k is loaded two periods before its use and scaled when stored.
+// Plain (cached) loads: the other half of each 128-byte line is the next chunk's, read one period
+// later from L2 (streaming loads that skip L2 refetched it from HBM: 25% slower).
 template <typename RT>
 struct Global16Producer {
   const RT *inc[4];
@@ -144,7 +145,7 @@ struct Global16Producer {
 #pragma unroll
       for (int p = 0; p < 4; ++p) {
         const RT *base = inc[p];
-        buf[p][e] = base ? __builtin_nontemporal_load(base + (int64_t)ro * ld + jo) : (RT)0;
+        buf[p][e] = base ? base[(int64_t)ro * ld + jo] : (RT)0;
       }
     }
     cl.advance(g);

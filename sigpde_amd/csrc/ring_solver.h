@@ -255,7 +255,7 @@ struct GlobalProducer {
       int ro = (b * G + rho) >> g.order;
       ro = ro > g.m - 1 ? g.m - 1 : ro;
       const RT *base = inc[pg];
-      buf[i] = base ? __builtin_nontemporal_load(base + (int64_t)ro * ld + jo) : (RT)0;
+      buf[i] = base ? base[(int64_t)ro * ld + jo] : (RT)0;
     }
     cl.advance(g);
   }
