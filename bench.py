@@ -19,7 +19,7 @@ Besides the JSON contract fields the line carries:
   latency_ms     p50 / p99 of the per-step times
   roofline       dominant kernel: achieved vs peak, measured with HIP events on its stream
   cpu_baseline   the oracle (C restatement, float32 reference semantics) timed on this host's
-                 cores on a sampled block, in pairs/sec
+                 cores on a sampled block, in pairs/sec (and on one thread: value_1_thread)
   workloads      the smaller workloads of the API, each with its float64-oracle error: float64
                  pairwise (8 pairs, length 32, dim 4, order 1: per-call latency p50 / p99), float32
                  Gram 128 x 128 (length 100, dim 8, order 2: every precision), float64
@@ -243,6 +243,13 @@ def cpu_leg(args, X, Y, K, others):
     t0 = time.perf_counter()
     oracle.linear_gram(X32[:side2], Y32[:side2], args.order)
     dt2 = time.perf_counter() - t0
+    # one thread, on a block of about a tenth of the time
+    oracle.set_num_threads(1)
+    side1 = int(max(4, min(X.shape[0], np.sqrt(side2 * side2 * 0.1 / cores))))
+    t0 = time.perf_counter()
+    oracle.linear_gram(X32[:side1], Y32[:side1], args.order)
+    dt1 = time.perf_counter() - t0
+    oracle.set_num_threads(cores)
     return {
         "rel_err_fp64": rel,
         "rel_err_fp64_max": float(per.max()),
@@ -255,6 +262,8 @@ def cpu_leg(args, X, Y, K, others):
         "kind": "port",
         "sample": f"{side2}x{side2} pairs of the same workload, float32 reference semantics "
                   f"(C restatement, OpenMP over pairs), {dt2:.1f} s",
+        "value_1_thread": side1 * side1 / dt1,
+        "sample_1_thread": f"{side1}x{side1} pairs on one thread, {dt1:.1f} s",
     }
 
 
