@@ -1,0 +1,151 @@
+// Goursat PDE wavefront of ONE pair on W waves of a workgroup (latency path for few, long pairs).
+//
+// Same scheme and cell arithmetic as ring_solver.h / ring16.h (stojl/SigPDE
+// sigpde/cuda_device_functions.py:3-5, sigpde/cuda_pairwise_kernels.py:8-39): lane rho of a wave
+// owns refined row i = 64 b + rho + 1 of band b and handles column v = tau - rho + 1 at band step
+// tau; the value above arrives from lane rho - 1 by DPP (wave_shr:1), and the same coefs / cell
+// functions round every cell as the other solvers do, so a pair's result is the same bit for bit.
+//
+// What differs is that the bands of one pair run concurrently on the W waves, pipelined:
+//   * Band b runs on wave b mod W during periods (16 steps each) [b D, b D + P), P = ceil((N+63)/16)
+//     periods per band. Every wave steps through the same global period counter and the workgroup
+//     synchronises once per period, so the schedule is static: no flags, no spinning.
+//   * Band b's last row reaches band b+1's first lane through an LDS window owned by the consumer
+//     wave: lane 63 writes column v at index v mod win, lane 0 reads it D periods later. D >= 5
+//     covers the 63-step skew of a band (the value of column v is written in period
+//     b D + floor((v + 62) / 16) and read in period (b+1) D + floor((v - 1) / 16)); win = 16 (D + 2)
+//     entries keep a value until it is read; (W - 1) D >= P - 4 makes a window free again (its
+//     reader done) before the band W later starts writing into it.
+//   * Increments are read straight from global memory (cached: a path's increments are reused by
+//     every root-find iteration), a period ahead of their use, 16 per lane per period.
+// Sequential steps per pair: ((nb - 1) D + P) 16 instead of nb P 16 on one wave (nb = ceil(M/64)).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ring16.h"
+#include "ring_solver.h"
+
+namespace sigpde {
+
+constexpr int kMwWaves = 8;  // waves per pair (two per SIMD of one CU)
+
+struct MwGeom {
+  int m, n, order;  // original increments (rows / columns) and dyadic order
+  int M, N;         // refined rows / columns
+  int nb;           // bands of 64 rows
+  int P;            // periods per band
+  int D;            // periods between band starts
+  int win;          // window entries
+  int T;            // periods of the whole pair
+  __host__ __device__ MwGeom(int m_, int n_, int o, int W) : m(m_), n(n_), order(o) {
+    M = m << o;
+    N = n << o;
+    nb = (M + 63) / 64;
+    P = (N + 63 + 15) / 16;
+    const int need = (P + 1 + (W - 2)) / (W - 1);  // ceil((P + 1) / (W - 1))
+    D = need > 5 ? need : 5;
+    win = 16 * (D + 2);
+    T = (nb - 1) * D + P;
+  }
+};
+
+// LDS of one workgroup: W windows of the band-boundary type, then one double (the result).
+template <Arith A>
+__host__ __device__ inline size_t mw_lds_bytes(const MwGeom &g, int W) {
+  using BT = typename R16Bnd<A>::T;
+  const size_t w = (size_t)W * g.win * sizeof(BT);
+  return (w + 7) / 8 * 8 + sizeof(double);
+}
+
+// Solves one pair: increments inc (m x n, row stride ld), pair scale `scale` applied to each
+// increment in RT when `scaled` (as the other producers do), Scaled* modes' per-call factor s.
+// Every thread of the workgroup (64 W threads) must call it; all get K[M][N].
+template <Arith A, int W>
+__device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, int64_t ld,
+                           typename ArithTraits<A>::RT scale, bool scaled, const MwGeom &g,
+                           double s, void *lds) {
+  using RT = typename ArithTraits<A>::RT;
+  using BT = typename R16Bnd<A>::T;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  BT *wins = reinterpret_cast<BT *>(lds);
+  double *res = reinterpret_cast<double *>(static_cast<char *>(lds) +
+                                           ((size_t)W * g.win * sizeof(BT) + 7) / 8 * 8);
+  BT *inWin = wins + wave * g.win;                    // band b-1 -> this wave's band b
+  BT *outWin = wins + ((wave + 1) % W) * g.win;       // this wave's band b -> band b+1
+  const int jo_max = g.n - 1;
+
+  double k = 1.0, kPrev = 1.0, result = 0.0;
+  RT cc[16], cn[16];
+  // increments of period q of band b, for this lane: columns v = 16 q + st - lane + 1
+  auto fetch = [&](RT (&dst)[16], int b, int q) {
+    int io = ((64 * b + lane) >> g.order);
+    io = io < g.m ? io : g.m - 1;
+    const RT *row = inc + (int64_t)io * ld;
+#pragma unroll
+    for (int st = 0; st < 16; ++st) {
+      int jo = (16 * q + st - lane) >> g.order;  // (v - 1) >> order
+      jo = jo < 0 ? 0 : (jo > jo_max ? jo_max : jo);
+      const RT c = row[jo];
+      dst[st] = scaled ? (RT)(c * scale) : c;
+    }
+  };
+
+  for (int p = 0; p < g.T; ++p) {
+    // this wave's band in period p (at most one: W D >= P + 1)
+    int b = -1, q = 0;
+    const int rel = p - wave * g.D;
+    if (rel >= 0) {
+      const int bb = wave + W * (rel / (W * g.D));
+      const int qq = p - bb * g.D;
+      if (bb < g.nb && qq < g.P) {
+        b = bb;
+        q = qq;
+      }
+    }
+    if (b >= 0) {
+      if (q == 0) {
+        k = 1.0;      // k[i][0]
+        kPrev = 1.0;  // k[i-1][0]
+        fetch(cc, b, 0);
+      } else {
+#pragma unroll
+        for (int st = 0; st < 16; ++st) cc[st] = cn[st];
+      }
+      if (q + 1 < g.P) fetch(cn, b, q + 1);
+      const int i = 64 * b + lane + 1;  // this lane's row (1-based)
+      // window index of column v = 16 q + st - lane + 1 is (v0 + st) mod win (win is a multiple
+      // of 16, v0 in [0, win))
+      int v0 = (16 * q - lane + 1) % g.win;
+      v0 = v0 < 0 ? v0 + g.win : v0;
+#pragma unroll
+      for (int st = 0; st < 16; ++st) {
+        const int v = 16 * q + st - lane + 1;  // this lane's column (1-based)
+        const int wi = v0 + st < g.win ? v0 + st : v0 + st - g.win;
+        // lane 0: the row above the band (ones for the first band and left of column 1)
+        BT top = (BT)1;
+        if (lane == 0 && b > 0 && v >= 1 && v <= g.N) top = inWin[wi];
+        const double up = shift_down<64>(k, (double)top);
+        if (v >= 1 && v <= g.N && i <= g.M) {
+          double cA, cB;
+          coefs<A>(cc[st], s, 0.0, 0.0, cA, cB);
+          k = cell<A>(up, k, kPrev, cA, cB);
+          if (lane == 63 && b + 1 < g.nb) outWin[wi] = (BT)k;
+          if (i == g.M && v == g.N) result = k;
+        }
+        kPrev = up;
+      }
+    }
+    __syncthreads();
+  }
+  // the corner lane publishes K[M][N]
+  if (64 * (g.nb - 1) + lane + 1 == g.M && wave == (g.nb - 1) % W) *res = result;
+  __syncthreads();
+  const double r = *res;
+  __syncthreads();  // the next call may overwrite the slot
+  return r;
+}
+
+}  // namespace sigpde

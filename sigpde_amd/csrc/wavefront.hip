@@ -10,6 +10,7 @@
 #include "capi_util.h"
 #include <type_traits>
 
+#include "mw_solver.h"
 #include "ring16.h"
 #include "ring_solver.h"
 
@@ -101,6 +102,19 @@ __global__ __launch_bounds__(kWave) void k_pde_pairwise(PairwiseArgs<T> a, int64
   });
 }
 
+// One pair per workgroup of W waves (mw_solver.h), for few long pairs.
+template <typename T, int W>
+__global__ __launch_bounds__(64 * W) void k_pde_pairwise_mw(PairwiseArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int64_t b = blockIdx.x;
+  const MwGeom g(a.m, a.n, a.order, W);
+  const bool scaled = a.sx || a.sy;
+  const T scale = scaled ? (a.sy ? a.sy[b] : (T)1) * (a.sx ? a.sx[b] : (T)1) : (T)1;
+  const double r = mw_solve<RefArith<T>::kPlain, W>(a.inc + b * a.stride_b, a.n, scale, scaled,
+                                                    g, 1.0, smem_raw);
+  if (threadIdx.x == 0) a.out[b] = (T)r;
+}
+
 // Pair p of a Gram tile -> (i, j). Symmetric diagonal tiles (off_x == off_y, bx == by) list only
 // the lower triangle j <= i; other tiles list all bx*by pairs (symmetric ones skip j above i).
 __device__ __forceinline__ void tile_pair(int64_t p, int64_t by, bool tri, int64_t &i, int64_t &j) {
@@ -187,6 +201,64 @@ __device__ __forceinline__ double log_normalizer(double x) {
   return 2.0 - 1.0 / (1.0 + log(x));
 }
 
+// Chandrupatla's bracketing iteration for one path (sigpde/cuda_pairwise_kernels.py:93-166):
+// F(x) = PDE(x^2 inc) - target on [0, 1], F(0) = 1 - target, F(1) = K(x,x) - target, the
+// bookkeeping and the interpolation test operation for operation, in double; function values
+// rounded through the path dtype T as the reference's scratch stores them. Stops when the smaller
+// bracket end's |F| < tol (function value only) or after maxit evaluations.
+template <typename T>
+struct RootFind {
+  double av, cv, bv, fa, fb, fc, t, outv;
+  int it;
+  T Tt;
+  __device__ void init(T K1, T target) {
+    Tt = target;
+    av = 0.0;
+    cv = 0.0;
+    bv = 1.0;
+    fa = 1.0 - (double)Tt;
+    fb = (double)(T)(K1 - Tt);
+    fc = fa;
+    t = 0.5;
+    outv = 0.0;
+    it = 0;
+  }
+  __device__ double next() const { return av + t * (bv - av); }
+  // Takes F's evaluation at xt = next() (the wavefront result sol); true when the search stops.
+  __device__ bool update(double xt, double sol, double tol, int maxit) {
+    ++it;
+    T solT = (T)sol;
+    double ft = isnan(solT) ? (double)INFINITY : (double)(T)(solT - Tt);
+    if (ft * fa >= 0) {
+      cv = av;
+      fc = fa;
+    } else {
+      cv = bv;
+      bv = av;
+      fc = fb;
+      fb = fa;
+    }
+    av = xt;
+    fa = ft;
+    bool stop;
+    if (fabs(fa) < fabs(fb)) {
+      outv = av;
+      stop = fabs(fa) < tol;
+    } else {
+      outv = bv;
+      stop = fabs(fb) < tol;
+    }
+    double xi = (av - bv) / (cv - bv);
+    double phi = (fa - fb) / (fc - fb);
+    if (phi * phi < xi && (1 - phi) * (1 - phi) < 1 - xi)
+      t = fa / (fb - fa) * fc / (fb - fc) + (cv - av) / (bv - av) * fa / (fc - fa) * fb / (fc - fb);
+    else
+      t = 0.5;
+    if (!(0 < t && t < 1)) t = 0.5;
+    return stop || it >= maxit;
+  }
+};
+
 // Broadcasts the value held by lane (group base + src) to every lane of its group of G lanes.
 template <int G>
 __device__ __forceinline__ double group_bcast(double v, int src) {
@@ -234,56 +306,56 @@ __global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a, int64_t 
     Tt = a.target[b];
   }
 
-  double av = 0.0, cv = 0.0, bv = 1.0;
-  double fa = 1.0 - (double)Tt;
-  double fb = (double)(T)(K1 - Tt);
-  double fc = fa, t = 0.5;
-  double outv = 0.0;
-  int it = 0;
+  RootFind<T> rf;
+  rf.init(K1, Tt);
   bool active = live && a.maxit > 0;
   while (__any(active)) {
-    double xt = av + t * (bv - av);
+    const double xt = rf.next();
     double r = 0.0;
     solve_wave<RefArith<T>::kScaled, G, R, GL>(
         P, g, xt * xt, smem_raw, wave_lines<RefArith<T>::kScaled, GL>(gl, g.N), [&](int, double v) { r = v; });
-    double sol = group_bcast<G>(r, rM);
+    const double sol = group_bcast<G>(r, rM);
     if (!active) continue;
-    ++it;
-    T solT = (T)sol;
-    double ft = isnan(solT) ? (double)INFINITY : (double)(T)(solT - Tt);
-    if (ft * fa >= 0) {
-      cv = av;
-      fc = fa;
-    } else {
-      cv = bv;
-      bv = av;
-      fc = fb;
-      fb = fa;
-    }
-    av = xt;
-    fa = ft;
-    bool stop;
-    if (fabs(fa) < fabs(fb)) {
-      outv = av;
-      stop = fabs(fa) < a.tol;
-    } else {
-      outv = bv;
-      stop = fabs(fb) < a.tol;
-    }
-    double xi = (av - bv) / (cv - bv);
-    double phi = (fa - fb) / (fc - fb);
-    if (phi * phi < xi && (1 - phi) * (1 - phi) < 1 - xi)
-      t = fa / (fb - fa) * fc / (fb - fc) + (cv - av) / (bv - av) * fa / (fc - fa) * fb / (fc - fb);
-    else
-      t = 0.5;
-    if (!(0 < t && t < 1)) t = 0.5;
-    if (stop || it >= a.maxit) active = false;
+    if (rf.update(xt, sol, a.tol, a.maxit)) active = false;
   }
   if (live && (lane & (G - 1)) == 0) {
     // scales[norms < 1] = 0 (sigpde/torch/kernel.py:195)
-    a.lambda[b] = (K1 < (T)1) ? (T)0 : (T)outv;
+    a.lambda[b] = (K1 < (T)1) ? (T)0 : (T)rf.outv;
     if (a.normalizer == 0) a.knorm[b] = K1;
-    if (a.iters) a.iters[b] = it;
+    if (a.iters) a.iters[b] = rf.it;
+  }
+}
+
+// The same root-find with one path per workgroup of W waves (mw_solver.h: the bands of each
+// wavefront evaluation run concurrently), for few long paths.
+template <typename T, int W>
+__global__ __launch_bounds__(64 * W) void k_robust_norm_mw(RobustArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int64_t b = blockIdx.x;
+  const MwGeom g(a.m, a.m, a.order, W);
+  const T *inc = a.inc + b * a.stride_b;
+  T K1 = (T)1, Tt = (T)1;
+  if (a.normalizer == 0) {
+    K1 = (T)mw_solve<RefArith<T>::kPlain, W>(inc, a.m, (T)1, false, g, 1.0, smem_raw);
+    if (isnan(K1)) K1 = (T)INFINITY;
+    Tt = log_normalizer<T>(K1);
+  } else {
+    K1 = a.knorm[b];
+    Tt = a.target[b];
+  }
+  RootFind<T> rf;
+  rf.init(K1, Tt);
+  bool active = a.maxit > 0;  // uniform over the workgroup: one path
+  while (active) {
+    const double xt = rf.next();
+    const double sol = mw_solve<RefArith<T>::kScaled, W>(inc, a.m, (T)1, false, g, xt * xt,
+                                                        smem_raw);
+    if (rf.update(xt, sol, a.tol, a.maxit)) active = false;
+  }
+  if (threadIdx.x == 0) {
+    a.lambda[b] = (K1 < (T)1) ? (T)0 : (T)rf.outv;
+    if (a.normalizer == 0) a.knorm[b] = K1;
+    if (a.iters) a.iters[b] = rf.it;
   }
 }
 
@@ -343,9 +415,28 @@ static hipError_t launch16(int order, int N, int64_t waves, bool global, hipStre
   return e != hipSuccess ? e : ef;
 }
 
+// Few long pairs run one pair per workgroup of kMwWaves waves (mw_solver.h): up to kMwMaxPairs
+// pairs (the GPU then still holds every workgroup at once) of at least kMwMinBands bands of 64
+// refined rows (where pipelining the bands shortens the sequential chain), when the windows fit
+// LDS. Returns the workgroup's LDS bytes, or 0 when the pairs take the other routes.
+constexpr int64_t kMwMaxPairs = 256;
+constexpr int kMwMinBands = 2 * kMwWaves;
+
+template <typename T>
+static size_t mw_route(int64_t pairs, int m, int n, int order) {
+  const MwGeom g(m, n, order, kMwWaves);
+  if (pairs > kMwMaxPairs || g.nb < kMwMinBands) return 0;
+  const size_t lds = mw_lds_bytes<RefArith<T>::kPlain>(g, kMwWaves);
+  return lds <= kMaxLdsBytes ? lds : 0;
+}
+
 template <typename T>
 static hipError_t launch_pairwise(const PairwiseArgs<T> &a, hipStream_t st) {
   constexpr Arith A = RefArith<T>::kPlain;
+  if (const size_t lds = mw_route<T>(a.B, a.m, a.n, a.order)) {
+    k_pde_pairwise_mw<T, kMwWaves><<<dim3((unsigned)a.B), dim3(64 * kMwWaves), lds, st>>>(a);
+    return hipGetLastError();
+  }
   const int N = a.n << a.order;
   const bool f16 = g16_fits<T>(N), f64 = g64_fits<T>(N);
   if ((a.B >= kManyPairs && f16) || !f64) {
@@ -384,6 +475,10 @@ static hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st) {
 template <typename T>
 static hipError_t launch_robust(const RobustArgs<T> &a, hipStream_t st) {
   constexpr Arith A = RefArith<T>::kPlain;
+  if (const size_t lds = mw_route<T>(a.B, a.m, a.m, a.order)) {
+    k_robust_norm_mw<T, kMwWaves><<<dim3((unsigned)a.B), dim3(64 * kMwWaves), lds, st>>>(a);
+    return hipGetLastError();
+  }
   const int N = a.m << a.order;
   const bool f16 = g16_fits<T>(N), f64 = g64_fits<T>(N);
   if ((a.B >= kManyPairs && f16) || !f64) {

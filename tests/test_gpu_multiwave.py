@@ -1,0 +1,81 @@
+"""GPU parity of the multi-wave route (sigpde_amd/csrc/mw_solver.h): up to 256 pairs or paths with
+at least 16 bands of 64 refined rows run one pair per workgroup of 8 waves, the bands of each pair
+pipelined across the waves. Checked against the oracle, and bit for bit against the one-wave-per-pair
+route (the same pairs in a batch of 257, above the multi-wave limit): the cell arithmetic is shared,
+so the route must not change a result.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from helpers import max_rel, rel_fro
+from sigpde_amd import _native
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+TOL = {torch.float64: 1e-10, torch.float32: 1e-5}
+NP = {torch.float64: np.float64, torch.float32: np.float32}
+
+
+def rand_inc(shape, seed, dtype, order=0):
+    rng = np.random.default_rng(seed)
+    m, n = shape[-2], shape[-1]
+    return (rng.normal(size=shape) * (1.5 / np.sqrt(m * n)) / 4.0 ** order).astype(NP[dtype])
+
+
+def solve(inc, order, sx=None, sy=None):
+    out = torch.zeros(inc.shape[0], dtype=inc.dtype, device=DEV)
+    _native.pde_pairwise(inc, order, out, sx, sy)
+    return out
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("B,m,n,order", [
+    (4, 1100, 40, 0),    # 18 bands, short rows of columns
+    (3, 300, 300, 2),    # 1200 x 1200 refined
+    (2, 2000, 50, 0),    # 32 bands
+    (5, 129, 2000, 3),   # 1032 x 16000 refined: the largest windows
+    (1, 1024, 7, 0),     # exactly 16 bands, tiny band length
+])
+def test_pairwise_multiwave(dtype, B, m, n, order):
+    inc = rand_inc((B, m, n), 3 * m + n + order, dtype, order)
+    ref = O.pde_pairwise(inc, order)
+    got = solve(torch.from_numpy(inc).to(DEV), order).cpu().numpy()
+    assert rel_fro(got, ref) < TOL[dtype]
+    assert max_rel(got, ref, floor=1e-3) < 10 * TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_multiwave_route_equals_one_wave_route_bitwise(dtype):
+    B, m, n, order = 3, 1030, 33, 1
+    inc = torch.from_numpy(rand_inc((B, m, n), 5, dtype, order)).to(DEV)
+    sx = torch.linspace(0.5, 1.0, B, dtype=dtype, device=DEV)
+    sy = torch.linspace(0.9, 0.6, B, dtype=dtype, device=DEV)
+    few = solve(inc, order, sx, sy)                        # multi-wave route
+    many_inc = inc.repeat(86, 1, 1)[:257].contiguous()     # 257 pairs: one wave per pair
+    many = solve(many_inc, order, sx.repeat(86)[:257].contiguous(),
+                 sy.repeat(86)[:257].contiguous())
+    assert torch.equal(few, many[:B])
+
+
+def test_robust_norm_multiwave():
+    B, m, order = 2, 1100, 0
+    rng = np.random.default_rng(17)
+    dx = rng.normal(size=(B, m, 3)) * (1.3 / np.sqrt(m))
+    inc = np.einsum("bsk,btk->bst", dx, dx)
+    lam_ref, k_ref, it_ref = O.norm_factors(inc, order, tol=1e-10, maxit=100)
+    lam = torch.zeros(B, dtype=torch.float64, device=DEV)
+    kn = torch.zeros(B, dtype=torch.float64, device=DEV)
+    iters = torch.zeros(B, dtype=torch.int32, device=DEV)
+    t = torch.from_numpy(inc).to(DEV)
+    _native.robust_norm(t, order, lam, kn, None, maxit=100, tol=1e-10, iters=iters)
+    assert rel_fro(kn.cpu().numpy(), k_ref) < 1e-10
+    assert max_rel(lam.cpu().numpy(), lam_ref, floor=1e-3) < 1e-9
+    assert np.array_equal(iters.cpu().numpy(), it_ref)
+    # bit for bit against the one-wave route (257 paths)
+    many = t.repeat(129, 1, 1)[:257].contiguous()
+    lam2 = torch.zeros(257, dtype=torch.float64, device=DEV)
+    kn2 = torch.zeros(257, dtype=torch.float64, device=DEV)
+    _native.robust_norm(many, order, lam2, kn2, None, maxit=100, tol=1e-10)
+    assert torch.equal(lam, lam2[:B]) and torch.equal(kn, kn2[:B])
