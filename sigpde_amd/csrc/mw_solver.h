@@ -29,8 +29,6 @@
 
 namespace sigpde {
 
-constexpr int kMwWaves = 8;  // waves per pair (two per SIMD of one CU)
-
 struct MwGeom {
   int m, n, order;  // original increments (rows / columns) and dyadic order
   int M, N;         // refined rows / columns

@@ -415,26 +415,33 @@ static hipError_t launch16(int order, int N, int64_t waves, bool global, hipStre
   return e != hipSuccess ? e : ef;
 }
 
-// Few long pairs run one pair per workgroup of kMwWaves waves (mw_solver.h): up to kMwMaxPairs
-// pairs (the GPU then still holds every workgroup at once) of at least kMwMinBands bands of 64
-// refined rows (where pipelining the bands shortens the sequential chain), when the windows fit
-// LDS. Returns the workgroup's LDS bytes, or 0 when the pairs take the other routes.
-constexpr int64_t kMwMaxPairs = 256;
-constexpr int kMwMinBands = 2 * kMwWaves;
-
+// Few long pairs run one pair per workgroup of W waves (mw_solver.h): W = 16 for up to 128 pairs
+// of at least 32 bands of 64 refined rows, W = 8 for up to 256 pairs of at least 16 bands (the GPU
+// then still holds every workgroup at once, and pipelining the bands shortens the sequential
+// chain), when the windows fit LDS. Sets W and returns the workgroup's LDS bytes, or returns 0
+// when the pairs take the other routes.
 template <typename T>
-static size_t mw_route(int64_t pairs, int m, int n, int order) {
-  const MwGeom g(m, n, order, kMwWaves);
-  if (pairs > kMwMaxPairs || g.nb < kMwMinBands) return 0;
-  const size_t lds = mw_lds_bytes<RefArith<T>::kPlain>(g, kMwWaves);
-  return lds <= kMaxLdsBytes ? lds : 0;
+static size_t mw_route(int64_t pairs, int m, int n, int order, int &W) {
+  for (const int w : {16, 8}) {
+    const MwGeom g(m, n, order, w);
+    if (pairs * w > 2048 || g.nb < 2 * w) continue;
+    const size_t lds = mw_lds_bytes<RefArith<T>::kPlain>(g, w);
+    if (lds > kMaxLdsBytes) continue;
+    W = w;
+    return lds;
+  }
+  return 0;
 }
 
 template <typename T>
 static hipError_t launch_pairwise(const PairwiseArgs<T> &a, hipStream_t st) {
   constexpr Arith A = RefArith<T>::kPlain;
-  if (const size_t lds = mw_route<T>(a.B, a.m, a.n, a.order)) {
-    k_pde_pairwise_mw<T, kMwWaves><<<dim3((unsigned)a.B), dim3(64 * kMwWaves), lds, st>>>(a);
+  int W = 0;
+  if (const size_t lds = mw_route<T>(a.B, a.m, a.n, a.order, W)) {
+    if (W == 16)
+      k_pde_pairwise_mw<T, 16><<<dim3((unsigned)a.B), dim3(64 * 16), lds, st>>>(a);
+    else
+      k_pde_pairwise_mw<T, 8><<<dim3((unsigned)a.B), dim3(64 * 8), lds, st>>>(a);
     return hipGetLastError();
   }
   const int N = a.n << a.order;
@@ -475,8 +482,12 @@ static hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st) {
 template <typename T>
 static hipError_t launch_robust(const RobustArgs<T> &a, hipStream_t st) {
   constexpr Arith A = RefArith<T>::kPlain;
-  if (const size_t lds = mw_route<T>(a.B, a.m, a.m, a.order)) {
-    k_robust_norm_mw<T, kMwWaves><<<dim3((unsigned)a.B), dim3(64 * kMwWaves), lds, st>>>(a);
+  int W = 0;
+  if (const size_t lds = mw_route<T>(a.B, a.m, a.m, a.order, W)) {
+    if (W == 16)
+      k_robust_norm_mw<T, 16><<<dim3((unsigned)a.B), dim3(64 * 16), lds, st>>>(a);
+    else
+      k_robust_norm_mw<T, 8><<<dim3((unsigned)a.B), dim3(64 * 8), lds, st>>>(a);
     return hipGetLastError();
   }
   const int N = a.m << a.order;

@@ -1,6 +1,6 @@
-"""GPU parity of the multi-wave route (sigpde_amd/csrc/mw_solver.h): up to 256 pairs or paths with
-at least 16 bands of 64 refined rows run one pair per workgroup of 8 waves, the bands of each pair
-pipelined across the waves. Checked against the oracle, and bit for bit against the one-wave-per-pair
+"""GPU parity of the multi-wave route (sigpde_amd/csrc/mw_solver.h): few pairs or paths with many
+bands of 64 refined rows run one pair per workgroup of 8 waves (up to 256 pairs, >= 16 bands) or
+16 waves (up to 128 pairs, >= 32 bands), the bands of each pair pipelined across the waves. Checked against the oracle, and bit for bit against the one-wave-per-pair
 route (the same pairs in a batch of 257, above the multi-wave limit): the cell arithmetic is shared,
 so the route must not change a result.
 """
@@ -32,9 +32,10 @@ def solve(inc, order, sx=None, sy=None):
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("B,m,n,order", [
-    (4, 1100, 40, 0),    # 18 bands, short rows of columns
+    (4, 1100, 40, 0),    # 18 bands (8 waves), short rows of columns
+    (130, 1100, 9, 0),   # 130 pairs: 8 waves each (16 would exceed the GPU's resident waves)
     (3, 300, 300, 2),    # 1200 x 1200 refined
-    (2, 2000, 50, 0),    # 32 bands
+    (2, 2000, 50, 0),    # 32 bands (16 waves)
     (5, 129, 2000, 3),   # 1032 x 16000 refined: the largest windows
     (1, 1024, 7, 0),     # exactly 16 bands, tiny band length
 ])
@@ -59,8 +60,9 @@ def test_multiwave_route_equals_one_wave_route_bitwise(dtype):
     assert torch.equal(few, many[:B])
 
 
-def test_robust_norm_multiwave():
-    B, m, order = 2, 1100, 0
+@pytest.mark.parametrize("m", [1100, 2100])  # 8 and 16 waves per path
+def test_robust_norm_multiwave(m):
+    B, order = 2, 0
     rng = np.random.default_rng(17)
     dx = rng.normal(size=(B, m, 3)) * (1.3 / np.sqrt(m))
     inc = np.einsum("bsk,btk->bst", dx, dx)
