@@ -81,3 +81,21 @@ def test_robust_norm_multiwave(m):
     kn2 = torch.zeros(257, dtype=torch.float64, device=DEV)
     _native.robust_norm(many, order, lam2, kn2, None, maxit=100, tol=1e-10)
     assert torch.equal(lam, lam2[:B]) and torch.equal(kn, kn2[:B])
+
+
+def test_robust_api_on_long_paths():
+    """RobustSigPDE end to end on few long float64 paths (the multi-wave route for both the
+    root-find and the scaled pairwise solve) against the oracle."""
+    import sigpde_amd.torch as sig
+    from helpers import brownian_np
+    x = brownian_np(3, 600, 3, 4.0 / 600, 31)
+    y = brownian_np(3, 600, 3, 4.0 / 600, 32)
+    rk = sig.RobustSigPDE(sig.kernels.LinearKernel(), 1)
+    xc, yc = torch.from_numpy(x).to(DEV), torch.from_numpy(y).to(DEV)
+    lx = rk.normalization(xc, tol=1e-10).cpu().numpy()
+    lx_ref = O.robust_normalization(x, 1, tol=1e-10)
+    assert np.max(np.abs(lx - lx_ref)) < 1e-9
+    got = rk.pairwise(xc, yc, tol=1e-10).cpu().numpy()
+    ly_ref = O.robust_normalization(y, 1, tol=1e-10)
+    ref = O.sig_pairwise(x, y, 1, sx=lx_ref, sy=ly_ref)
+    assert rel_fro(got, ref) < 1e-9
