@@ -4,7 +4,8 @@
 //   inc[i][j] = (((k[i+1][j+1] + k[i][j]) - k[i+1][j]) - k[i][j+1]) * 4^-order
 // The reference materialises the point Gram and then about ten full-size temporaries (the
 // distance, its scaling, exp, three slice differences, the division); here a tile of increments
-// (32 x 128 from inner products: rows of 129 contiguous reads and 128 contiguous writes; 16 x 16
+// (32 x 255 from inner products: a thread per point column, rows of 256 contiguous reads and 255
+// contiguous writes; 16 x 16
 // for L1-RBF, whose cost is the distance sums) reads its point values once, keeps k in LDS and
 // writes the increments once.
 //   linear: k = kscale * <x,y> with <x,y> the caller's GEMM output (the reference scheme's
@@ -26,7 +27,8 @@
 
 namespace sigpde {
 
-constexpr int kIpRows = 32, kIpCols = 128;  // inner-product route: increments per tile
+constexpr int kIpRows = 32, kIpCols = 255;  // inner-product route: increments per tile (a thread
+                                             // per point column: 256 = the block)
 constexpr int kL1Tile = 16;                  // L1-RBF: increments per tile side
 constexpr int kDimChunk = 32;                // L1: dimensions staged in LDS at a time
 
@@ -53,6 +55,7 @@ struct IncGeom {
 template <typename T, int TR, int TC, int KS>
 __device__ __forceinline__ void write_tile(const T *kt, const IncGeom &g, int64_t a, int64_t b,
                                            int i0, int j0, T inv4, T *out) {
+  if (threadIdx.x >= TC * (256 / TC)) return;
   const int lj = threadIdx.x % TC;
   const int j = j0 + lj;
   if (j >= g.n) return;
@@ -76,20 +79,25 @@ __global__ __launch_bounds__(256) void k_ip_increments(const T *__restrict__ ip,
                                                        int64_t yb, IncGeom g, T c, T inv4,
                                                        T *__restrict__ out) {
   constexpr int KR = kIpRows + 1, KC = kIpCols + 1;
+  static_assert(KC == 256, "a thread per point column");
   __shared__ T kt[KR * KC];
   int64_t a, b;
   int i0, j0;
   g.tile(blockIdx.x, a, b, i0, j0);
-  for (int p = threadIdx.x; p < KR * KC; p += blockDim.x) {
-    const int li = p / KC, lj = p % KC;
-    const int i = min(i0 + li, g.m), j = min(j0 + lj, g.n);  // clamped: unused past the edge
-    const T v = ip[a * sa + b * sb + i * si + j * sj];
+  const int lj = threadIdx.x;
+  const int j = min(j0 + lj, g.n);  // clamped: unused past the edge
+  const T *col = ip + a * sa + b * sb + j * sj;
+  T yv = (T)0;
+  if constexpr (RBF) yv = yy[a * ya + b * yb + j];
+  for (int li = 0; li < KR; ++li) {
+    const int i = min(i0 + li, g.m);
+    const T v = col[i * si];
     if constexpr (RBF) {
       T dist = (T)-2 * v;
-      dist += xx[a * xa + i] + yy[a * ya + b * yb + j];
-      kt[p] = exp((-dist) * c);
+      dist += xx[a * xa + i] + yv;
+      kt[li * KC + lj] = exp((-dist) * c);
     } else {
-      kt[p] = c * v;
+      kt[li * KC + lj] = c * v;
     }
   }
   __syncthreads();
