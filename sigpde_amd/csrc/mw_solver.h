@@ -118,22 +118,59 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
       // of 16, v0 in [0, win))
       int v0 = (16 * q - lane + 1) % g.win;
       v0 = v0 < 0 ? v0 + g.win : v0;
+      // lane 0's row above the band for the period's columns 16 q + 1 .. 16 q + 16, all written
+      // in earlier periods (D >= 5): read (broadcast to every lane) four at a time, a group of
+      // four steps ahead of their use (ones for the first band and past column N)
+      const int t0 = (16 * q + 1) % g.win;
+      auto tops4 = [&](BT (&t)[4], int grp) {
 #pragma unroll
-      for (int st = 0; st < 16; ++st) {
-        const int v = 16 * q + st - lane + 1;  // this lane's column (1-based)
-        const int wi = v0 + st < g.win ? v0 + st : v0 + st - g.win;
-        // lane 0: the row above the band (ones for the first band and left of column 1)
-        BT top = (BT)1;
-        if (lane == 0 && b > 0 && v >= 1 && v <= g.N) top = inWin[wi];
-        const double up = shift_down<64>(k, (double)top);
-        if (v >= 1 && v <= g.N && i <= g.M) {
+        for (int e = 0; e < 4; ++e) {
+          const int st = 4 * grp + e;
+          const int wt = t0 + st < g.win ? t0 + st : t0 + st - g.win;
+          t[e] = (b > 0 && 16 * q + st + 1 <= g.N) ? inWin[wt] : (BT)1;
+        }
+      };
+      BT tcur[4], tnxt[4];
+      tops4(tcur, 0);
+      const bool hand = lane == 63 && b + 1 < g.nb;  // this lane feeds the next band
+      // Steady periods: every lane's 16 columns lie inside 1 .. N - 1 and every row inside the
+      // grid, so no step needs a range test (and none holds the corner, column N).
+      const bool steady = 16 * q >= 63 && 16 * q + 16 < g.N && 64 * b + 64 <= g.M;
+      if (steady) {
+#pragma unroll
+        for (int st = 0; st < 16; ++st) {
+          if (st % 4 == 0 && st < 12) tops4(tnxt, st / 4 + 1);
+          const double up = shift_down<64>(k, (double)tcur[st % 4]);
+          if (st % 4 == 3 && st < 15) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) tcur[e] = tnxt[e];
+          }
           double cA, cB;
           coefs<A>(cc[st], s, 0.0, 0.0, cA, cB);
           k = cell<A>(up, k, kPrev, cA, cB);
-          if (lane == 63 && b + 1 < g.nb) outWin[wi] = (BT)k;
-          if (i == g.M && v == g.N) result = k;
+          if (hand) outWin[v0 + st < g.win ? v0 + st : v0 + st - g.win] = (BT)k;
+          kPrev = up;
         }
-        kPrev = up;
+      } else {
+#pragma unroll
+        for (int st = 0; st < 16; ++st) {
+          if (st % 4 == 0 && st < 12) tops4(tnxt, st / 4 + 1);
+          const int v = 16 * q + st - lane + 1;  // this lane's column (1-based)
+          const int wi = v0 + st < g.win ? v0 + st : v0 + st - g.win;
+          const double up = shift_down<64>(k, (double)tcur[st % 4]);
+          if (st % 4 == 3 && st < 15) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) tcur[e] = tnxt[e];
+          }
+          if (v >= 1 && v <= g.N && i <= g.M) {
+            double cA, cB;
+            coefs<A>(cc[st], s, 0.0, 0.0, cA, cB);
+            k = cell<A>(up, k, kPrev, cA, cB);
+            if (hand) outWin[wi] = (BT)k;
+            if (i == g.M && v == g.N) result = k;
+          }
+          kPrev = up;
+        }
       }
     }
     __syncthreads();

@@ -418,11 +418,13 @@ static hipError_t launch16(int order, int N, int64_t waves, bool global, hipStre
 // Few long pairs run one pair per workgroup of W waves (mw_solver.h): W = 16 for up to 128 pairs
 // of at least 32 bands of 64 refined rows, W = 8 for up to 256 pairs of at least 16 bands (the GPU
 // then still holds every workgroup at once, and pipelining the bands shortens the sequential
-// chain), when the windows fit LDS. Sets W and returns the workgroup's LDS bytes, or returns 0
-// when the pairs take the other routes.
+// chain), when the windows fit LDS. Float64 stays at W = 8 (at 16 waves per CU its 128 registers
+// per lane would spill). Sets W and returns the workgroup's LDS bytes, or returns 0 when the pairs
+// take the other routes.
 template <typename T>
 static size_t mw_route(int64_t pairs, int m, int n, int order, int &W) {
   for (const int w : {16, 8}) {
+    if (w == 16 && sizeof(T) == 8) continue;
     const MwGeom g(m, n, order, w);
     if (pairs * w > 2048 || g.nb < 2 * w) continue;
     const size_t lds = mw_lds_bytes<RefArith<T>::kPlain>(g, w);
@@ -438,10 +440,13 @@ static hipError_t launch_pairwise(const PairwiseArgs<T> &a, hipStream_t st) {
   constexpr Arith A = RefArith<T>::kPlain;
   int W = 0;
   if (const size_t lds = mw_route<T>(a.B, a.m, a.n, a.order, W)) {
-    if (W == 16)
-      k_pde_pairwise_mw<T, 16><<<dim3((unsigned)a.B), dim3(64 * 16), lds, st>>>(a);
-    else
-      k_pde_pairwise_mw<T, 8><<<dim3((unsigned)a.B), dim3(64 * 8), lds, st>>>(a);
+    if constexpr (sizeof(T) == 4) {
+      if (W == 16) {
+        k_pde_pairwise_mw<T, 16><<<dim3((unsigned)a.B), dim3(64 * 16), lds, st>>>(a);
+        return hipGetLastError();
+      }
+    }
+    k_pde_pairwise_mw<T, 8><<<dim3((unsigned)a.B), dim3(64 * 8), lds, st>>>(a);
     return hipGetLastError();
   }
   const int N = a.n << a.order;
@@ -484,10 +489,13 @@ static hipError_t launch_robust(const RobustArgs<T> &a, hipStream_t st) {
   constexpr Arith A = RefArith<T>::kPlain;
   int W = 0;
   if (const size_t lds = mw_route<T>(a.B, a.m, a.m, a.order, W)) {
-    if (W == 16)
-      k_robust_norm_mw<T, 16><<<dim3((unsigned)a.B), dim3(64 * 16), lds, st>>>(a);
-    else
-      k_robust_norm_mw<T, 8><<<dim3((unsigned)a.B), dim3(64 * 8), lds, st>>>(a);
+    if constexpr (sizeof(T) == 4) {
+      if (W == 16) {
+        k_robust_norm_mw<T, 16><<<dim3((unsigned)a.B), dim3(64 * 16), lds, st>>>(a);
+        return hipGetLastError();
+      }
+    }
+    k_robust_norm_mw<T, 8><<<dim3((unsigned)a.B), dim3(64 * 8), lds, st>>>(a);
     return hipGetLastError();
   }
   const int N = a.m << a.order;

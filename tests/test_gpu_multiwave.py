@@ -1,6 +1,7 @@
 """GPU parity of the multi-wave route (sigpde_amd/csrc/mw_solver.h): few pairs or paths with many
-bands of 64 refined rows run one pair per workgroup of 8 waves (up to 256 pairs, >= 16 bands) or
-16 waves (up to 128 pairs, >= 32 bands), the bands of each pair pipelined across the waves. Checked against the oracle, and bit for bit against the one-wave-per-pair
+bands of 64 refined rows run one pair per workgroup of 8 waves (up to 256 pairs, >= 16 bands) or,
+in float32, 16 waves (up to 128 pairs, >= 32 bands), the bands of each pair pipelined across the
+waves. Checked against the oracle, and bit for bit against the one-wave-per-pair
 route (the same pairs in a batch of 257, above the multi-wave limit): the cell arithmetic is shared,
 so the route must not change a result.
 """
