@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 #include "ring16.h"
 #include "ring_solver.h"
 
@@ -48,6 +50,43 @@ struct MwGeom {
     T = (nb - 1) * D + P;
   }
 };
+
+// The 16 consecutive values p[0..15] (p aligned to its element), as 16-byte vector loads.
+template <typename RT>
+__device__ __forceinline__ void mw_load_run16(const RT *p, RT (&dst)[16]) {
+  constexpr int E = 16 / sizeof(RT);
+  typedef RT vec_t __attribute__((ext_vector_type(E), aligned(sizeof(RT))));
+#pragma unroll
+  for (int k = 0; k < 16 / E; ++k) {
+    const vec_t v = *reinterpret_cast<const vec_t *>(p + E * k);
+#pragma unroll
+    for (int e = 0; e < E; ++e) dst[E * k + e] = v[e];
+  }
+}
+
+// dst[st] = row[(c0 + st) >> O] for st = 0..15 (O >= 1, c0 >= 0, row[(c0 + 15) >> O] in range):
+// the distinct values are loaded once each and repeated. Every index is a constant of a fold
+// expression (no loops), so the values stay in registers: a loop, before it is unrolled, would
+// let the compiler turn the per-lane choice between two values into an indexed scratch access.
+template <int O, typename RT, int... K, int... ST>
+__device__ __forceinline__ void mw_load_rep16(const RT *row, int c0, RT (&dst)[16],
+                                              std::integer_sequence<int, K...>,
+                                              std::integer_sequence<int, ST...>) {
+  constexpr int kMask = (1 << O) - 1;
+  constexpr int kCnt = sizeof...(K);
+  static_assert(kCnt == ((kMask + 15) >> O) + 1, "distinct values of 16 columns");
+  static_assert((15 >> O) + 1 < kCnt, "index of the last step's value");
+  const int j0 = c0 >> O, jl = (c0 + 15) >> O, ph = c0 & kMask;
+  const RT v[kCnt] = {row[j0 + K < jl ? j0 + K : jl]...};
+  // (ph + st) >> O = (st >> O) + 1 exactly when ph + (st & kMask) passes kMask
+  ((dst[ST] = ph + (ST & kMask) > kMask ? v[(ST >> O) + 1] : v[ST >> O]), ...);
+}
+
+template <int O, typename RT>
+__device__ __forceinline__ void mw_load_rep16(const RT *row, int c0, RT (&dst)[16]) {
+  mw_load_rep16<O>(row, c0, dst, std::make_integer_sequence<int, (((1 << O) + 14) >> O) + 1>(),
+                   std::make_integer_sequence<int, 16>());
+}
 
 // LDS of one workgroup: W windows of the band-boundary type, then one double (the result).
 template <Arith A>
@@ -77,17 +116,39 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
 
   double k = 1.0, kPrev = 1.0, result = 0.0;
   RT cc[16], cn[16];
-  // increments of period q of band b, for this lane: columns v = 16 q + st - lane + 1
+  // increments of period q of band b, for this lane: columns v = 16 q + st - lane + 1. When all
+  // 16 lie inside 1 .. N (16 q >= 63, 16 q + 16 <= N) they are one run of the lane's row: read as
+  // vector loads at order 0 and as the few distinct values at higher orders, so that a wave's
+  // load instructions touch each cache line a few times rather than once per column.
   auto fetch = [&](RT (&dst)[16], int b, int q) {
     int io = ((64 * b + lane) >> g.order);
     io = io < g.m ? io : g.m - 1;
     const RT *row = inc + (int64_t)io * ld;
+    const int c0 = 16 * q - lane;  // v - 1 of step 0
+    if (16 * q >= 63 && 16 * q + 16 <= g.N) {
+      switch (g.order) {
+        case 0: mw_load_run16(row + c0, dst); break;
+        case 1: mw_load_rep16<1>(row, c0, dst); break;
+        case 2: mw_load_rep16<2>(row, c0, dst); break;
+        case 3: mw_load_rep16<3>(row, c0, dst); break;
+        default: {  // 16 columns span at most two original columns
+          const int j0 = c0 >> g.order, jl = (c0 + 15) >> g.order;
+          const RT v0 = row[j0], v1 = row[jl];
 #pragma unroll
-    for (int st = 0; st < 16; ++st) {
-      int jo = (16 * q + st - lane) >> g.order;  // (v - 1) >> order
-      jo = jo < 0 ? 0 : (jo > jo_max ? jo_max : jo);
-      const RT c = row[jo];
-      dst[st] = scaled ? (RT)(c * scale) : c;
+          for (int st = 0; st < 16; ++st) dst[st] = ((c0 + st) >> g.order) == j0 ? v0 : v1;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int st = 0; st < 16; ++st) {
+        int jo = (c0 + st) >> g.order;
+        jo = jo < 0 ? 0 : (jo > jo_max ? jo_max : jo);
+        dst[st] = row[jo];
+      }
+    }
+    if (scaled) {
+#pragma unroll
+      for (int st = 0; st < 16; ++st) dst[st] = (RT)(dst[st] * scale);
     }
   };
 
