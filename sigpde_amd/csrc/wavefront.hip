@@ -415,20 +415,33 @@ static hipError_t launch16(int order, int N, int64_t waves, bool global, hipStre
   return e != hipSuccess ? e : ef;
 }
 
-// Few long pairs run one pair per workgroup of W waves (mw_solver.h): W = 16 for up to 128 pairs
-// of at least 32 bands of 64 refined rows, W = 8 for up to 256 pairs of at least 16 bands (the GPU
-// then still holds every workgroup at once, and pipelining the bands shortens the sequential
-// chain), when the windows fit LDS. Float64 stays at W = 8 (at 16 waves per CU its 128 registers
-// per lane would spill). Sets W and returns the workgroup's LDS bytes, or returns 0 when the pairs
-// take the other routes.
+// Workgroups of `kernel` (64 W threads, lds bytes of LDS) the current GPU holds at once.
+static int64_t resident_groups(const void *kernel, int W, size_t lds) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 64 * W, lds) != hipSuccess)
+    return 0;
+  return (int64_t)cus * per;
+}
+
+// Pairs of several bands of 64 refined rows run one pair per workgroup of W waves (mw_solver.h)
+// when the GPU holds all of their workgroups at once and the windows fit LDS: W = 16 for at least
+// 8 bands, else W = 8 for at least 4. Pipelining the bands over the waves shortens each pair's
+// sequential chain; with more pairs than resident workgroups the one-wave routes, which fill the
+// SIMDs with pairs instead, are faster. kern16 / kern8: the kernels of the two widths (kern16 null
+// for float64, whose registers would spill at 16 waves per CU). Sets W and returns the
+// workgroup's LDS bytes, or returns 0 when the pairs take the other routes.
 template <typename T>
-static size_t mw_route(int64_t pairs, int m, int n, int order, int &W) {
+static size_t mw_route(int64_t pairs, int m, int n, int order, const void *kern16,
+                       const void *kern8, int &W) {
   for (const int w : {16, 8}) {
-    if (w == 16 && sizeof(T) == 8) continue;
+    const void *kern = w == 16 ? kern16 : kern8;
+    if (!kern) continue;
     const MwGeom g(m, n, order, w);
-    if (pairs * w > 2048 || g.nb < 2 * w) continue;
+    if (g.nb < w / 2) continue;
     const size_t lds = mw_lds_bytes<RefArith<T>::kPlain>(g, w);
-    if (lds > kMaxLdsBytes) continue;
+    if (lds > kMaxLdsBytes || pairs > resident_groups(kern, w, lds)) continue;
     W = w;
     return lds;
   }
@@ -439,7 +452,10 @@ template <typename T>
 static hipError_t launch_pairwise(const PairwiseArgs<T> &a, hipStream_t st) {
   constexpr Arith A = RefArith<T>::kPlain;
   int W = 0;
-  if (const size_t lds = mw_route<T>(a.B, a.m, a.n, a.order, W)) {
+  const void *k16 = nullptr;
+  if constexpr (sizeof(T) == 4) k16 = reinterpret_cast<const void *>(&k_pde_pairwise_mw<T, 16>);
+  const void *k8 = reinterpret_cast<const void *>(&k_pde_pairwise_mw<T, 8>);
+  if (const size_t lds = mw_route<T>(a.B, a.m, a.n, a.order, k16, k8, W)) {
     if constexpr (sizeof(T) == 4) {
       if (W == 16) {
         k_pde_pairwise_mw<T, 16><<<dim3((unsigned)a.B), dim3(64 * 16), lds, st>>>(a);
@@ -488,7 +504,10 @@ template <typename T>
 static hipError_t launch_robust(const RobustArgs<T> &a, hipStream_t st) {
   constexpr Arith A = RefArith<T>::kPlain;
   int W = 0;
-  if (const size_t lds = mw_route<T>(a.B, a.m, a.m, a.order, W)) {
+  const void *k16 = nullptr;
+  if constexpr (sizeof(T) == 4) k16 = reinterpret_cast<const void *>(&k_robust_norm_mw<T, 16>);
+  const void *k8 = reinterpret_cast<const void *>(&k_robust_norm_mw<T, 8>);
+  if (const size_t lds = mw_route<T>(a.B, a.m, a.m, a.order, k16, k8, W)) {
     if constexpr (sizeof(T) == 4) {
       if (W == 16) {
         k_robust_norm_mw<T, 16><<<dim3((unsigned)a.B), dim3(64 * 16), lds, st>>>(a);

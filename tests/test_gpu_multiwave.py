@@ -1,9 +1,9 @@
-"""GPU parity of the multi-wave route (sigpde_amd/csrc/mw_solver.h): few pairs or paths with many
-bands of 64 refined rows run one pair per workgroup of 8 waves (up to 256 pairs, >= 16 bands) or,
-in float32, 16 waves (up to 128 pairs, >= 32 bands), the bands of each pair pipelined across the
-waves. Checked against the oracle, and bit for bit against the one-wave-per-pair
-route (the same pairs in a batch of 257, above the multi-wave limit): the cell arithmetic is shared,
-so the route must not change a result.
+"""GPU parity of the multi-wave route (sigpde_amd/csrc/mw_solver.h): pairs of at least 4 bands of
+64 refined rows run one pair per workgroup of 8 waves (16 in float32 from 8 bands), the bands of each
+pair pipelined across the waves, when the GPU holds all of the pairs' workgroups at once. Checked
+against the oracle, and bit for bit against the one-wave-per-pair route (the same pairs repeated
+past the GPU's resident workgroups): the cell arithmetic is shared, so the route must not change a
+result.
 """
 import numpy as np
 import pytest
@@ -33,8 +33,10 @@ def solve(inc, order, sx=None, sy=None):
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("B,m,n,order", [
-    (4, 1100, 40, 0),    # 18 bands (8 waves), short rows of columns
-    (130, 1100, 9, 0),   # 130 pairs: 8 waves each (16 would exceed the GPU's resident waves)
+    (4, 1100, 40, 0),    # 18 bands, short rows of columns
+    (130, 1100, 9, 0),   # 130 pairs
+    (6, 300, 50, 0),     # 5 bands: 8 waves in float32 too
+    (300, 600, 5, 0),    # 300 pairs of 10 bands (float32: two workgroups per CU)
     (3, 300, 300, 2),    # 1200 x 1200 refined
     (2, 2000, 50, 0),    # 32 bands (16 waves)
     (5, 129, 2000, 3),   # 1032 x 16000 refined: the largest windows
@@ -58,13 +60,16 @@ def test_multiwave_route_equals_one_wave_route_bitwise(dtype):
     sx = torch.linspace(0.5, 1.0, B, dtype=dtype, device=DEV)
     sy = torch.linspace(0.9, 0.6, B, dtype=dtype, device=DEV)
     few = solve(inc, order, sx, sy)                        # multi-wave route
-    many_inc = inc.repeat(86, 1, 1)[:257].contiguous()     # 257 pairs: one wave per pair
-    many = solve(many_inc, order, sx.repeat(86)[:257].contiguous(),
-                 sy.repeat(86)[:257].contiguous())
+    # more pairs than the GPU holds workgroups of 8 waves (at most four per CU): one wave per pair
+    nm = 4 * torch.cuda.get_device_properties(0).multi_processor_count + 1
+    reps = (nm + B - 1) // B
+    many_inc = inc.repeat(reps, 1, 1)[:nm].contiguous()
+    many = solve(many_inc, order, sx.repeat(reps)[:nm].contiguous(),
+                 sy.repeat(reps)[:nm].contiguous())
     assert torch.equal(few, many[:B])
 
 
-@pytest.mark.parametrize("m", [1100, 2100])  # 8 and 16 waves per path
+@pytest.mark.parametrize("m", [300, 1100, 2100])
 def test_robust_norm_multiwave(m):
     B, order = 2, 0
     rng = np.random.default_rng(17)
@@ -79,10 +84,12 @@ def test_robust_norm_multiwave(m):
     assert rel_fro(kn.cpu().numpy(), k_ref) < 1e-10
     assert max_rel(lam.cpu().numpy(), lam_ref, floor=1e-3) < 1e-9
     assert np.array_equal(iters.cpu().numpy(), it_ref)
-    # bit for bit against the one-wave route (257 paths)
-    many = t.repeat(129, 1, 1)[:257].contiguous()
-    lam2 = torch.zeros(257, dtype=torch.float64, device=DEV)
-    kn2 = torch.zeros(257, dtype=torch.float64, device=DEV)
+    # bit for bit against the one-wave route: more paths than the GPU holds workgroups of the
+    # float64 root-find (174 registers per lane: one workgroup of 8 waves per CU)
+    nm = 2 * torch.cuda.get_device_properties(0).multi_processor_count + 1
+    many = t.repeat((nm + B - 1) // B, 1, 1)[:nm].contiguous()
+    lam2 = torch.zeros(nm, dtype=torch.float64, device=DEV)
+    kn2 = torch.zeros(nm, dtype=torch.float64, device=DEV)
     _native.robust_norm(many, order, lam2, kn2, None, maxit=100, tol=1e-10)
     assert torch.equal(lam, lam2[:B]) and torch.equal(kn, kn2[:B])
 
