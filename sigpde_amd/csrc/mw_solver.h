@@ -11,7 +11,9 @@
 //     periods per band. Every wave steps through the same global period counter and the workgroup
 //     synchronises once per period, so the schedule is static: no flags, no spinning.
 //   * Band b's last row reaches band b+1's first lane through an LDS window owned by the consumer
-//     wave: lane 63 writes column v at index v mod win, lane 0 reads it D periods later. D >= 5
+//     wave: lane 63 writes column v at index (v + 63) mod win, lane 0 reads it D periods later
+//     (its 16 columns of a period are then one aligned block of 16 slots; the first band reads a
+//     window of ones, the boundary k[0][j] = 1, filled at the start of every solve). D >= 5
 //     covers the 63-step skew of a band (the value of column v is written in period
 //     b D + floor((v + 62) / 16) and read in period (b+1) D + floor((v - 1) / 16)); win = 16 (D + 2)
 //     entries keep a value until it is read; (W - 1) D >= P - 4 makes a window free again (its
@@ -115,6 +117,17 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
   const int jo_max = g.n - 1;
 
   double k = 1.0, kPrev = 1.0, result = 0.0;
+  // k of lane rho - 1 (lane 0: `top`, the band above). In the float32 modes every cell value is
+  // a float (cell rounds it), so the float is shifted: one DPP move instead of two, no conversion
+  // of the top value, the same value.
+  auto shift_up = [](double kk, BT top) -> double {
+    if constexpr (ArithTraits<A>::kRound) {
+      static_assert(sizeof(BT) == 4, "float32 modes keep float windows");
+      return (double)__int_as_float(dpp_down1<64>(__float_as_int((float)kk), __float_as_int(top)));
+    } else {
+      return shift_down<64>(kk, (double)top);
+    }
+  };
   RT cc[16], cn[16];
   // increments of period q of band b, for this lane: columns v = 16 q + st - lane + 1. When all
   // 16 lie inside 1 .. N (16 q >= 63, 16 q + 16 <= N) they are one run of the lane's row: read as
@@ -152,6 +165,11 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
     }
   };
 
+  // band 0 (wave 0) reads the boundary row of ones from its window; the band that writes this
+  // window next (band W - 1, from period (W - 1) D >= P) starts after band 0 has ended
+  for (int x = threadIdx.x; x < g.win; x += 64 * W) wins[x] = (BT)1;
+  __syncthreads();
+
   for (int p = 0; p < g.T; ++p) {
     // this wave's band in period p (at most one: W D >= P + 1)
     int b = -1, q = 0;
@@ -175,21 +193,20 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
       }
       if (q + 1 < g.P) fetch(cn, b, q + 1);
       const int i = 64 * b + lane + 1;  // this lane's row (1-based)
-      // window index of column v = 16 q + st - lane + 1 is (v0 + st) mod win (win is a multiple
-      // of 16, v0 in [0, win))
-      int v0 = (16 * q - lane + 1) % g.win;
-      v0 = v0 < 0 ? v0 + g.win : v0;
+      // lane 63 writes its columns 16 q + st - 62 at window slots wb + st + 1 (win is a multiple
+      // of 16: only the last may wrap)
+      const int wb = (16 * q) % g.win;
+      const int w15 = wb + 16 < g.win ? wb + 16 : 0;
       // lane 0's row above the band for the period's columns 16 q + 1 .. 16 q + 16, all written
-      // in earlier periods (D >= 5): read (broadcast to every lane) four at a time, a group of
-      // four steps ahead of their use (ones for the first band and past column N)
-      const int t0 = (16 * q + 1) % g.win;
+      // in earlier periods (D >= 5), at slots r0 .. r0 + 15: read (broadcast to every lane) four
+      // at a time, a group of four steps ahead of their use. Slots of columns past N hold stale
+      // values; they reach only lane 0's cells past N, which are not computed.
+      const int r0 = (16 * q + 64) % g.win;
       auto tops4 = [&](BT (&t)[4], int grp) {
+        typedef BT vec4 __attribute__((ext_vector_type(4), aligned(16)));
+        const vec4 v = *reinterpret_cast<const vec4 *>(inWin + r0 + 4 * grp);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int st = 4 * grp + e;
-          const int wt = t0 + st < g.win ? t0 + st : t0 + st - g.win;
-          t[e] = (b > 0 && 16 * q + st + 1 <= g.N) ? inWin[wt] : (BT)1;
-        }
+        for (int e = 0; e < 4; ++e) t[e] = v[e];
       };
       BT tcur[4], tnxt[4];
       tops4(tcur, 0);
@@ -201,7 +218,7 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
 #pragma unroll
         for (int st = 0; st < 16; ++st) {
           if (st % 4 == 0 && st < 12) tops4(tnxt, st / 4 + 1);
-          const double up = shift_down<64>(k, (double)tcur[st % 4]);
+          const double up = shift_up(k, tcur[st % 4]);
           if (st % 4 == 3 && st < 15) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) tcur[e] = tnxt[e];
@@ -209,7 +226,7 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
           double cA, cB;
           coefs<A>(cc[st], s, 0.0, 0.0, cA, cB);
           k = cell<A>(up, k, kPrev, cA, cB);
-          if (hand) outWin[v0 + st < g.win ? v0 + st : v0 + st - g.win] = (BT)k;
+          if (hand) outWin[st < 15 ? wb + st + 1 : w15] = (BT)k;
           kPrev = up;
         }
       } else {
@@ -217,8 +234,7 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
         for (int st = 0; st < 16; ++st) {
           if (st % 4 == 0 && st < 12) tops4(tnxt, st / 4 + 1);
           const int v = 16 * q + st - lane + 1;  // this lane's column (1-based)
-          const int wi = v0 + st < g.win ? v0 + st : v0 + st - g.win;
-          const double up = shift_down<64>(k, (double)tcur[st % 4]);
+          const double up = shift_up(k, tcur[st % 4]);
           if (st % 4 == 3 && st < 15) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) tcur[e] = tnxt[e];
@@ -227,7 +243,7 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
             double cA, cB;
             coefs<A>(cc[st], s, 0.0, 0.0, cA, cB);
             k = cell<A>(up, k, kPrev, cA, cB);
-            if (hand) outWin[wi] = (BT)k;
+            if (hand) outWin[st < 15 ? wb + st + 1 : w15] = (BT)k;
             if (i == g.M && v == g.N) result = k;
           }
           kPrev = up;

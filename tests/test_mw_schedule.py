@@ -67,7 +67,7 @@ def emulate(inc, o, W):
                 ups = np.empty(64)
                 for lane in range(64):
                     v = 16 * q + st - lane + 1
-                    wi = v % win
+                    wi = (v + 63) % win                 # lane 0's 16 columns: an aligned block
                     if lane == 0:
                         up = 1.0
                         if b > 0 and 1 <= v <= N:
