@@ -286,6 +286,15 @@ class RobustSigPDE:
         _check_paths(x)
         return self._normalization(x.contiguous(), normalizer, tol, maxit, max_batch)
 
+    @staticmethod
+    def _joint(x, y, normalizer):
+        """x's and y's root-finds can share one launch: paths of one shape and dtype, and the
+        default normaliser (elementwise; a custom one sees each batch as the reference passes it).
+        Each path's root-find is independent of the other paths in its launch; one launch keeps
+        more of the GPU busy when there are few paths."""
+        return (normalizer is log_normalizer and x.shape[1:] == y.shape[1:]
+                and x.dtype == y.dtype and x.device == y.device)
+
     def pairwise(self, x, y=None, normalizer=None, tol=1e-8, maxit=100, max_batch=1000, max_threads=1024):
         symmetric = y is None
         y = x if symmetric else y
@@ -296,17 +305,29 @@ class RobustSigPDE:
         x, y = x.contiguous(), y.contiguous()
         B = x.shape[0]
         out = torch.zeros(B, device=x.device, dtype=x.dtype)
+        joint = not symmetric and self._joint(x, y, normalizer)
         for start, stop in _chunks(B, int(max_batch)):
             xi = x[start:stop]
-            x_inc = pairwise_inner_product(xi, xi, self.static_kernel, self.dyadic_order).contiguous()
-            nb = stop - start
-            sx = torch.zeros(nb, device=x.device, dtype=x.dtype)
-            kx = torch.zeros(nb, device=x.device, dtype=x.dtype)
-            self._norm_factors(x_inc, sx, kx, normalizer, tol, maxit)
             yi = xi if symmetric else y[start:stop]
+            nb = stop - start
+            if joint:
+                xy = torch.cat([xi, yi])
+                xy_inc = pairwise_inner_product(xy, xy, self.static_kernel,
+                                                self.dyadic_order).contiguous()
+                sxy = torch.zeros(2 * nb, device=x.device, dtype=x.dtype)
+                kxy = torch.zeros(2 * nb, device=x.device, dtype=x.dtype)
+                self._norm_factors(xy_inc, sxy, kxy, normalizer, tol, maxit)
+                sx, sy = sxy[:nb], sxy[nb:]
+                x_inc = None
+            else:
+                x_inc = pairwise_inner_product(xi, xi, self.static_kernel,
+                                               self.dyadic_order).contiguous()
+                sx = torch.zeros(nb, device=x.device, dtype=x.dtype)
+                kx = torch.zeros(nb, device=x.device, dtype=x.dtype)
+                self._norm_factors(x_inc, sx, kx, normalizer, tol, maxit)
             if symmetric:
                 sy = sx
-            else:
+            elif not joint:
                 y_inc = pairwise_inner_product(yi, yi, self.static_kernel, self.dyadic_order).contiguous()
                 sy = torch.zeros(nb, device=x.device, dtype=x.dtype)
                 ky = torch.zeros(nb, device=x.device, dtype=x.dtype)
@@ -322,6 +343,13 @@ class RobustSigPDE:
 
     def gram(self, x, y=None, x_scale=None, y_scale=None, normalizer=None, tol=1e-8, maxit=100,
              max_batch=1000, max_threads=1024):
+        nz = self.normalizer if normalizer is None else normalizer
+        if y is not None and x_scale is None and y_scale is None:
+            _check_paths(x)
+            _check_paths(y)
+            if self._joint(x, y, nz):
+                lam = self._normalization(torch.cat([x, y]).contiguous(), nz, tol, maxit, max_batch)
+                x_scale, y_scale = lam[:x.shape[0]], lam[x.shape[0]:]
         x_scales = self.normalization(x, normalizer, tol, maxit, max_batch, max_threads) \
             if x_scale is None else x_scale
         if y is None:

@@ -131,6 +131,41 @@ def test_robust_pairwise_and_gram_f64():
     assert rel_fro(g, O.sig_gram(x, y, 1, sx=lx, sy=ly)) < 1e-9
 
 
+def test_robust_joint_root_find():
+    """x's and y's root-finds share one launch when the paths have one shape and the default
+    normaliser is used; otherwise (other lengths, a custom normaliser) they run apart. Both ways
+    against the oracle, and the shared launch against separate normalization calls."""
+    x = brownian_np(9, 24, 4, 0.15, 8)
+    y = brownian_np(9, 24, 4, 0.15, 9)
+    rk = sig.RobustSigPDE(sig.kernels.LinearKernel(), 1)
+    lx = rk.normalization(cuda(x), tol=1e-10)
+    ly = rk.normalization(cuda(y), tol=1e-10)
+    sep = sig.SigPDE(sig.kernels.LinearKernel(), 1).pairwise(cuda(x), cuda(y), lx, ly)
+    joint = rk.pairwise(cuda(x), cuda(y), tol=1e-10)
+    np.testing.assert_allclose(joint.cpu().numpy(), sep.cpu().numpy(), rtol=1e-12)
+    sep = sig.SigPDE(sig.kernels.LinearKernel(), 1).gram(cuda(x), cuda(y), lx, ly)
+    joint = rk.gram(cuda(x), cuda(y), tol=1e-10)
+    np.testing.assert_allclose(joint.cpu().numpy(), sep.cpu().numpy(), rtol=1e-12)
+    # other lengths: separate root-finds
+    y2 = brownian_np(9, 31, 4, 0.15, 10)
+    lx64 = O.robust_normalization(x, 1, tol=1e-10)
+    ly64 = O.robust_normalization(y2, 1, tol=1e-10)
+    got = rk.pairwise(cuda(x), cuda(y2), tol=1e-10).cpu().numpy()
+    assert rel_fro(got, O.sig_pairwise(x, y2, 1, sx=lx64, sy=ly64)) < 1e-9
+    got = rk.gram(cuda(x), cuda(y2), tol=1e-10).cpu().numpy()
+    assert rel_fro(got, O.sig_gram(x, y2, 1, sx=lx64, sy=ly64)) < 1e-9
+
+    # a custom normaliser sees x's and y's batches apart, as in the reference
+    def my_norm(v):
+        return 2 - 1 / v.pow(0.1) if isinstance(v, torch.Tensor) else 2 - 1 / np.power(v, 0.1)
+
+    inc = lambda p: O.second_difference(O.static_pairwise("linear", p, p), 1)  # noqa: E731
+    lxc, _, _ = O.norm_factors(inc(x), 1, my_norm, tol=1e-10)
+    lyc, _, _ = O.norm_factors(inc(y), 1, my_norm, tol=1e-10)
+    got = rk.pairwise(cuda(x), cuda(y), normalizer=my_norm, tol=1e-10).cpu().numpy()
+    assert rel_fro(got, O.sig_pairwise(x, y, 1, sx=lxc, sy=lyc)) < 1e-9
+
+
 @pytest.mark.parametrize("prec", ["fp16x2", "bf16x3", "fp32", "fp64"])
 def test_robust_pairwise_and_gram_fused(prec):
     """Robust final solves on the fused linear-kernel kernels (float32 inputs): the lambdas come from
