@@ -123,7 +123,7 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
   auto shift_up = [](double kk, BT top) -> double {
     if constexpr (ArithTraits<A>::kRound) {
       static_assert(sizeof(BT) == 4, "float32 modes keep float windows");
-      return (double)__int_as_float(dpp_down1<64>(__float_as_int((float)kk), __float_as_int(top)));
+      return shift_down_f<64>(kk, top);
     } else {
       return shift_down<64>(kk, (double)top);
     }

@@ -247,7 +247,11 @@ __device__ __forceinline__ void r16_group(const char *rb, const R16Period<typena
     const int sidx = 4 * I + k;
     double cA, cB;
     coefs<A>(cur.c[k], s, st.h, st.q, cA, cB);
-    const double up = shift_down<16>(st.k[R - 1], (double)cur.bv[k]);
+    double up;
+    if constexpr (ArithTraits<A>::kRound)  // float cells and float lines
+      up = shift_down_f<16>(st.k[R - 1], cur.bv[k]);
+    else
+      up = shift_down<16>(st.k[R - 1], (double)cur.bv[k]);
     bool z = false;
     if constexpr (F::kReset) z = pp.rsi == sidx;
     double above = up, aboveOld = st.kPrev;

@@ -71,6 +71,12 @@ __device__ __forceinline__ double shift_down(double v, double old) {
   r.y = dpp_down1<G>(a.y, o.y);
   return *reinterpret_cast<double *>(&r);
 }
+// Float form of shift_down for the float32 modes (Arith with kRound), whose cell values are all
+// floats: the float is shifted (one DPP move instead of two; `top` a float too), the same value.
+template <int G>
+__device__ __forceinline__ double shift_down_f(double v, float old) {
+  return (double)__int_as_float(dpp_down1<G>(__float_as_int((float)v), __float_as_int(old)));
+}
 __device__ __forceinline__ double readlane_d(double v, int l) {
   int2 a = *reinterpret_cast<int2 *>(&v);
   int2 r;
