@@ -469,7 +469,7 @@ def main():
     compare = not args.no_compare
     K32, ms32, _ = (timed(args, X, Y, "fp32", args.steps, 1) if args.mode != "fp32" and compare
                     else (K, ms, step_ms))
-    Kref, ms_ref, _ = (timed(args, X, Y, "ref", args.ref_steps, 0) if args.mode != "ref" and compare
+    Kref, ms_ref, _ = (timed(args, X, Y, "ref", args.ref_steps, 1) if args.mode != "ref" and compare
                        else (K, ms, step_ms))
     Kb, msb, _ = (timed(args, X, Y, "bf16x3", args.steps, 1) if args.mode != "bf16x3" and compare
                   else (K, ms, step_ms))

@@ -427,9 +427,9 @@ static int64_t resident_groups(const void *kernel, int W, size_t lds) {
 
 // Pairs of several bands of 64 refined rows run one pair per workgroup of W waves (mw_solver.h)
 // when the GPU holds all of their workgroups at once and the windows fit LDS: W = 16 for at least
-// 8 bands, else W = 8 for at least 4. Pipelining the bands over the waves shortens each pair's
-// sequential chain; with more pairs than resident workgroups the one-wave routes, which fill the
-// SIMDs with pairs instead, are faster. kern16 / kern8: the kernels of the two widths (kern16 null
+// 8 bands, else W = 8 for at least 4, and bands longer than the D periods between band starts.
+// Pipelining the bands over the waves shortens each pair's sequential chain; with more pairs than
+// resident workgroups the one-wave routes, which fill the SIMDs with pairs instead, are faster. kern16 / kern8: the kernels of the two widths (kern16 null
 // for float64, whose registers would spill at 16 waves per CU). Sets W and returns the
 // workgroup's LDS bytes, or returns 0 when the pairs take the other routes.
 template <typename T>
@@ -439,7 +439,7 @@ static size_t mw_route(int64_t pairs, int m, int n, int order, const void *kern1
     const void *kern = w == 16 ? kern16 : kern8;
     if (!kern) continue;
     const MwGeom g(m, n, order, w);
-    if (g.nb < w / 2) continue;
+    if (g.nb < w / 2 || g.D >= g.P) continue;  // D >= P: bands would not overlap
     const size_t lds = mw_lds_bytes<RefArith<T>::kPlain>(g, w);
     if (lds > kMaxLdsBytes || pairs > resident_groups(kern, w, lds)) continue;
     W = w;

@@ -40,7 +40,8 @@ def solve(inc, order, sx=None, sy=None):
     (3, 300, 300, 2),    # 1200 x 1200 refined
     (2, 2000, 50, 0),    # 32 bands (16 waves)
     (5, 129, 2000, 3),   # 1032 x 16000 refined: the largest windows
-    (1, 1024, 7, 0),     # exactly 16 bands, tiny band length
+    (1, 1024, 30, 0),    # exactly 16 bands, short bands (6 periods)
+    (1, 1024, 7, 0),     # 5 periods a band: no overlap, the one-wave route
     (2, 600, 200, 1),    # order 1: 9 distinct increments per lane and period
     (2, 70, 40, 4),      # order 4: 16 columns span two increments
     (1, 40, 20, 5),      # order 5
