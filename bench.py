@@ -29,7 +29,7 @@ Besides the JSON contract fields the line carries:
                  the request mix (70% SigPDE / 30% RobustSigPDE, pairwise or Gram, batch 1-256,
                  length 16-512, dim 4/16/64, dyadic order 0-3, float32 Brownian paths, 64 GB HBM
                  budget): saturated throughput (all queued at once), then Poisson arrivals at 70%
-                 of it: p50 / p99 request latency beside pairs/sec
+                 of it: p50 / p99 request latency beside pairs/sec (sweep: at 50% and 90%)
 """
 from __future__ import annotations
 
@@ -413,7 +413,8 @@ def serving_leg(args, dev):
     """Serves a bounded sample of the request mix; every SigPDE and robust final solve runs in
     precision fp16x2 (the robust root-find in the reference's arithmetic on the float32
     increments). First the whole sample queued at once (saturated throughput), then the same
-    requests with Poisson arrivals at 70% of that rate (or at --serving-rate): p50 / p99 latency."""
+    requests with Poisson arrivals at 70% of that rate (or at --serving-rate): p50 / p99 latency;
+    `sweep` repeats the Poisson run at 50% and 90% of the saturated rate."""
     from sigpde_amd import serving as S
 
     warm = S.synthetic_requests(6, seed=99, device=dev, precision="fp16x2", max_batch_size=16,
@@ -434,7 +435,13 @@ def serving_leg(args, dev):
     sat, _ = serve(float("inf"))
     rate = args.serving_rate if args.serving_rate else 0.7 * sat["requests_per_s"]
     out, launches = serve(rate)
-    out.update({"rate_per_s": rate, "launches": launches,
+    sweep = []
+    for frac in (0.5, 0.9):
+        r = frac * sat["requests_per_s"]
+        o, _ = serve(r)
+        sweep.append({"load": frac, "rate_per_s": r, "requests_per_s": o["requests_per_s"],
+                      "pairs_per_s": o["pairs_per_s"], "latency_ms": o["latency_ms"]})
+    out.update({"rate_per_s": rate, "launches": launches, "sweep": sweep,
                 "saturated": {"requests_per_s": sat["requests_per_s"],
                               "pairs_per_s": sat["pairs_per_s"], "wall_s": sat["wall_s"],
                               "latency_ms": sat["latency_ms"]},
