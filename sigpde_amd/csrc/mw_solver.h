@@ -16,10 +16,12 @@
 //     window of ones, the boundary k[0][j] = 1, filled at the start of every solve). D >= 5
 //     covers the 63-step skew of a band (the value of column v is written in period
 //     b D + floor((v + 62) / 16) and read in period (b+1) D + floor((v - 1) / 16)); win = 16 (D + 2)
-//     entries keep a value until it is read; (W - 1) D >= P - 4 makes a window free again (its
-//     reader done) before the band W later starts writing into it.
+//     entries keep a value until it is read; (W - 1) D >= P + 1 (D's definition) makes a window
+//     free again (its reader done; band 0's window of ones included) before the band W later
+//     starts writing into it.
 //   * Increments are read straight from global memory (cached: a path's increments are reused by
-//     every root-find iteration), a period ahead of their use, 16 per lane per period.
+//     every root-find iteration), a period ahead of their use: a run of 16 columns of the lane's
+//     row per period.
 // Sequential steps per pair: ((nb - 1) D + P) 16 instead of nb P 16 on one wave (nb = ceil(M/64)).
 #pragma once
 
