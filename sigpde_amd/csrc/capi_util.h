@@ -30,6 +30,24 @@ __attribute__((format(printf, 2, 3))) inline int set_error(int code, const char 
   return code;
 }
 
+// Workgroups of `kernel` (threads per workgroup, lds bytes of LDS) one CU of the current device
+// holds at once (0 when the query fails).
+inline int blocks_per_cu(const void *kernel, int threads, size_t lds) {
+  int per = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, lds) != hipSuccess)
+    return 0;
+  return per;
+}
+
+// CUs of the current device (0 when the query fails).
+inline int cu_count() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  return cus;
+}
+
 inline int hip_status(hipError_t e, const char *what) {
   if (e == hipSuccess) return SIGPDE_OK;
   return set_error(e == hipErrorOutOfMemory ? SIGPDE_ENOMEM : SIGPDE_EDEVICE, "%s: %s", what,

@@ -110,23 +110,17 @@ __global__ __launch_bounds__(256) void k_split_f64(const T *__restrict__ X, int 
 }
 
 // Arithmetic of the wavefront for each producer precision.
-template <int PREC>
-struct FusedArith {
-  static constexpr Arith A =
-      PREC == kPrecFp32 ? Arith::Ref32 : (PREC == kPrecFp64 ? Arith::Ref64 : Arith::Mixed);
-};
-
 // ------------------------------------------------------------------ kernels
 // Gram: a wave takes four x paths (a quad) and streams `ys` y paths through them.
 // Symmetric: only y <= x pairs are written, with their mirror.
 
 // Occupancy: at dim <= 32 (one K block) the fp16x2 kernel fits three waves per SIMD (measured 9%
 // faster at dim 16); the other instantiations would spill there and keep the compiler's choice.
-template <typename TO, int KB, int PREC, int R>
+// GL: boundary lines in global memory at gl (one block per wave of the launch), else in LDS.
+template <typename TO, int KB, int PREC, int R, bool GL>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(KB == 1 && PREC == kPrecFp16x2 && R == 1 ? 3 : 1))) void
-k_linear_gram_fused(GramFusedArgs<TO> a,
-                                                          const double *sxd, const double *syd) {
+k_linear_gram_fused(GramFusedArgs<TO> a, const double *sxd, const double *syd, void *gl) {
   using Prod = MfmaProducer<KB, 4, 1, PREC>;
   using E = typename Prod::elt;
   constexpr Arith A = FusedArith<PREC>::A;
@@ -185,7 +179,7 @@ k_linear_gram_fused(GramFusedArgs<TO> a,
     P.xsl = Prod::kSplit ? a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0) : 1.0;
     P.pmY = 1.0;
   }
-  ring16_solve<A, R>(P, geo, 1.0, smem_raw, [&](int sp, double r) {
+  ring16_solve<A, R, GL>(P, geo, 1.0, smem_raw, [&](int sp, double r) {
     const int64_t y = y0 + sp;
     if (xme >= a.Bx || y >= a.By) return;
     if (a.symmetric) {
@@ -195,7 +189,7 @@ k_linear_gram_fused(GramFusedArgs<TO> a,
     } else {
       a.out[xme * a.ldo + y] = (TO)r;
     }
-  });
+  }, fused_wave_lines<A, GL>(gl, geo.N));
 }
 
 // double copies of optional scales
@@ -313,10 +307,8 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
   a.ldo = ldo;
   const double *sxd = sx ? w.sxd : nullptr;
   const double *syd = sy ? w.syd : nullptr;
-  // Ref32 and Mixed: float ring and lines; Ref64 (fp64): double
-  const size_t shmem = r16_lds_bytes<FusedArith<PREC>::A>((int)((Ly - 1) << order));
   if (pairwise) {
-    return launch_linear_pairwise_fused<T, KB, PREC>(a, sxd, syd, shmem, st);
+    return launch_linear_pairwise_fused<T, KB, PREC>(a, sxd, syd, st);
   } else {
     // y paths streamed per wave: enough waves to fill the GPU, long enough streams to amortise
     // the 15-step pipeline fill of each pair
@@ -337,14 +329,28 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
     // a multiple of 8 blocks so that the XCD-major block order covers every work unit
     const int64_t grid = (nblk + 7) / 8 * 8;
     const dim3 g3((unsigned)grid), b3(64);
+    constexpr Arith A = FusedArith<PREC>::A;
+    const int N = (int)((Ly - 1) << order);
+    auto go = [&](auto rc) -> hipError_t {
+      constexpr int R = decltype(rc)::value;
+      bool gl = false;
+      const size_t shmem = fused_line_placement<A>(
+          reinterpret_cast<const void *>(&k_linear_gram_fused<T, KB, PREC, R, false>),
+          reinterpret_cast<const void *>(&k_linear_gram_fused<T, KB, PREC, R, true>), N, grid, gl);
+      return with_fused_lines(gl, (size_t)grid * r16_line_bytes<A>(N), st, [&](void *lines) {
+        if (lines)
+          k_linear_gram_fused<T, KB, PREC, R, true><<<g3, b3, shmem, st>>>(a, sxd, syd, lines);
+        else
+          k_linear_gram_fused<T, KB, PREC, R, false><<<g3, b3, shmem, st>>>(a, sxd, syd, nullptr);
+      });
+    };
     switch (r16_rows(order)) {
-      case 1: k_linear_gram_fused<T, KB, PREC, 1><<<g3, b3, shmem, st>>>(a, sxd, syd); break;
-      case 2: k_linear_gram_fused<T, KB, PREC, 2><<<g3, b3, shmem, st>>>(a, sxd, syd); break;
-      case 4: k_linear_gram_fused<T, KB, PREC, 4><<<g3, b3, shmem, st>>>(a, sxd, syd); break;
-      default: k_linear_gram_fused<T, KB, PREC, 8><<<g3, b3, shmem, st>>>(a, sxd, syd); break;
+      case 1: return go(std::integral_constant<int, 1>());
+      case 2: return go(std::integral_constant<int, 2>());
+      case 4: return go(std::integral_constant<int, 4>());
+      default: return go(std::integral_constant<int, 8>());
     }
   }
-  return hipGetLastError();
 }
 
 template <typename T, int PREC>

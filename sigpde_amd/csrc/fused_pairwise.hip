@@ -6,15 +6,15 @@
 
 namespace sigpde {
 
-// Pairwise: a wave solves four pairs (x_b, y_b).
-template <typename TO, int KB, int PREC, int R>
+// Pairwise: a wave solves four pairs (x_b, y_b). GL: boundary lines in global memory at gl (one
+// block per wave of the launch), else in LDS.
+template <typename TO, int KB, int PREC, int R, bool GL>
 __global__ __launch_bounds__(64) void k_linear_pairwise_fused(GramFusedArgs<TO> a,
                                                               const double *sxd,
-                                                              const double *syd) {
+                                                              const double *syd, void *gl) {
   using Prod = MfmaProducer<KB, 4, 4, PREC>;
   using E = typename Prod::elt;
-  constexpr Arith A =
-      PREC == kPrecFp32 ? Arith::Ref32 : (PREC == kPrecFp64 ? Arith::Ref64 : Arith::Mixed);
+  constexpr Arith A = FusedArith<PREC>::A;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int G = 16;
   const int lane = threadIdx.x & 63;
@@ -48,31 +48,49 @@ __global__ __launch_bounds__(64) void k_linear_pairwise_fused(GramFusedArgs<TO> 
     P.xsl = Prod::kSplit ? a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0) : 1.0;
     P.pmY = 1.0;  // the y scale is read when the pair's first chunk is multiplied
   }
-  ring16_solve<A, R>(P, geo, 1.0, smem_raw, [&](int, double r) {
+  ring16_solve<A, R, GL>(P, geo, 1.0, smem_raw, [&](int, double r) {
     if (bme < a.Bx) a.out[bme] = (TO)r;
-  });
+  }, fused_wave_lines<A, GL>(gl, geo.N));
 }
 
 template <typename T, int KB, int PREC>
 hipError_t launch_linear_pairwise_fused(const GramFusedArgs<T> &a, const double *sxd,
-                                        const double *syd, size_t shmem, hipStream_t st) {
-  const dim3 grid((unsigned)((a.Bx + 3) / 4)), block(64);
+                                        const double *syd, hipStream_t st) {
+  constexpr Arith A = FusedArith<PREC>::A;
+  const int64_t waves = (a.Bx + 3) / 4;
+  const dim3 grid((unsigned)waves), block(64);
+  const int N = a.n << a.order;
+  auto go = [&](auto rc) -> hipError_t {
+    constexpr int R = decltype(rc)::value;
+    bool gl = false;
+    const size_t shmem = fused_line_placement<A>(
+        reinterpret_cast<const void *>(&k_linear_pairwise_fused<T, KB, PREC, R, false>),
+        reinterpret_cast<const void *>(&k_linear_pairwise_fused<T, KB, PREC, R, true>), N, waves,
+        gl);
+    return with_fused_lines(gl, (size_t)waves * r16_line_bytes<A>(N), st, [&](void *lines) {
+      if (lines)
+        k_linear_pairwise_fused<T, KB, PREC, R, true><<<grid, block, shmem, st>>>(a, sxd, syd,
+                                                                                  lines);
+      else
+        k_linear_pairwise_fused<T, KB, PREC, R, false><<<grid, block, shmem, st>>>(a, sxd, syd,
+                                                                                   nullptr);
+    });
+  };
   switch (r16_rows(a.order)) {
-    case 1: k_linear_pairwise_fused<T, KB, PREC, 1><<<grid, block, shmem, st>>>(a, sxd, syd); break;
-    case 2: k_linear_pairwise_fused<T, KB, PREC, 2><<<grid, block, shmem, st>>>(a, sxd, syd); break;
-    case 4: k_linear_pairwise_fused<T, KB, PREC, 4><<<grid, block, shmem, st>>>(a, sxd, syd); break;
-    default: k_linear_pairwise_fused<T, KB, PREC, 8><<<grid, block, shmem, st>>>(a, sxd, syd); break;
+    case 1: return go(std::integral_constant<int, 1>());
+    case 2: return go(std::integral_constant<int, 2>());
+    case 4: return go(std::integral_constant<int, 4>());
+    default: return go(std::integral_constant<int, 8>());
   }
-  return hipGetLastError();
 }
 
 #define SIGPDE_PAIRWISE_INST(T, KB)                                                               \
   template hipError_t launch_linear_pairwise_fused<T, KB, kPrecFp16x2>(                          \
-      const GramFusedArgs<T> &, const double *, const double *, size_t, hipStream_t);          \
+      const GramFusedArgs<T> &, const double *, const double *, hipStream_t);          \
   template hipError_t launch_linear_pairwise_fused<T, KB, kPrecFp32>(                            \
-      const GramFusedArgs<T> &, const double *, const double *, size_t, hipStream_t);         \
+      const GramFusedArgs<T> &, const double *, const double *, hipStream_t);         \
   template hipError_t launch_linear_pairwise_fused<T, KB, kPrecBf16x3>(                          \
-      const GramFusedArgs<T> &, const double *, const double *, size_t, hipStream_t);
+      const GramFusedArgs<T> &, const double *, const double *, hipStream_t);
 SIGPDE_PAIRWISE_INST(float, 1)
 SIGPDE_PAIRWISE_INST(float, 2)
 SIGPDE_PAIRWISE_INST(float, 3)
@@ -83,12 +101,12 @@ SIGPDE_PAIRWISE_INST(double, 3)
 SIGPDE_PAIRWISE_INST(double, 4)
 #undef SIGPDE_PAIRWISE_INST
 template hipError_t launch_linear_pairwise_fused<float, 1, kPrecFp64>(
-    const GramFusedArgs<float> &, const double *, const double *, size_t, hipStream_t);
+    const GramFusedArgs<float> &, const double *, const double *, hipStream_t);
 template hipError_t launch_linear_pairwise_fused<float, 2, kPrecFp64>(
-    const GramFusedArgs<float> &, const double *, const double *, size_t, hipStream_t);
+    const GramFusedArgs<float> &, const double *, const double *, hipStream_t);
 template hipError_t launch_linear_pairwise_fused<double, 1, kPrecFp64>(
-    const GramFusedArgs<double> &, const double *, const double *, size_t, hipStream_t);
+    const GramFusedArgs<double> &, const double *, const double *, hipStream_t);
 template hipError_t launch_linear_pairwise_fused<double, 2, kPrecFp64>(
-    const GramFusedArgs<double> &, const double *, const double *, size_t, hipStream_t);
+    const GramFusedArgs<double> &, const double *, const double *, hipStream_t);
 
 }  // namespace sigpde

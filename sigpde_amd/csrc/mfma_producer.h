@@ -7,6 +7,7 @@
 
 #include <type_traits>
 
+#include "capi_util.h"
 #include "ring16.h"
 
 namespace sigpde {
@@ -276,11 +277,63 @@ struct GramFusedArgs {
   int64_t ldo;
 };
 
+// Arithmetic of the wavefront for a precision mode: float32 typing for fp32, double for fp64,
+// the mixed double wavefront on float increment products for the split 16-bit modes.
+template <int PREC>
+struct FusedArith {
+  static constexpr Arith A =
+      PREC == kPrecFp32 ? Arith::Ref32 : (PREC == kPrecFp64 ? Arith::Ref64 : Arith::Mixed);
+};
+
+// Where a fused launch keeps its band-boundary lines: in LDS (r16_lds_bytes) unless lines that
+// long would let fewer waves share a CU than the kernel's registers allow (long rows of
+// columns), then in global memory, one block of r16_line_bytes per wave of the launch (at most
+// kFusedLineBytes in all), with only the ring in LDS. Results are the same either way.
+// k_lds / k_gl: the kernel of each placement. Returns the launch's LDS bytes; sets gl.
+constexpr size_t kFusedLineBytes = (size_t)4 << 30;
+template <Arith A>
+inline size_t fused_line_placement(const void *k_lds, const void *k_gl, int N, int64_t waves,
+                                   bool &gl) {
+  const size_t lds_lines = r16_lds_bytes<A>(N), lds_ring = r16_ring_bytes<A>();
+  gl = false;
+  if ((double)waves * (double)r16_line_bytes<A>(N) > (double)kFusedLineBytes) return lds_lines;
+  const int occ_lds = lds_lines <= kMaxLdsBytes ? blocks_per_cu(k_lds, 64, lds_lines) : 0;
+  gl = blocks_per_cu(k_gl, 64, lds_ring) > occ_lds;
+  return gl ? lds_ring : lds_lines;
+}
+
+// launch(lines): lines null, or (gl) a block of `bytes` allocated stream-ordered on st for the
+// launch and freed after it on the same stream.
+template <class F>
+inline hipError_t with_fused_lines(bool gl, size_t bytes, hipStream_t st, F &&launch) {
+  if (!gl) {
+    launch(nullptr);
+    return hipGetLastError();
+  }
+  void *lines = nullptr;
+  hipError_t e = hipMallocAsync(&lines, bytes, st);
+  if (e != hipSuccess) return e;
+  launch(lines);
+  e = hipGetLastError();
+  const hipError_t ef = hipFreeAsync(lines, st);
+  return e != hipSuccess ? e : ef;
+}
+
+// This wave's block of global boundary lines (GL launches), else null.
+template <Arith A, bool GL>
+__device__ __forceinline__ typename R16Bnd<A>::T *fused_wave_lines(void *gl, int N) {
+  if constexpr (GL)
+    return reinterpret_cast<typename R16Bnd<A>::T *>(static_cast<char *>(gl) +
+                                                     (size_t)blockIdx.x * r16_line_bytes<A>(N));
+  else
+    return nullptr;
+}
+
 // Pairwise launcher, defined in fused_pairwise.hip (a translation unit of its own: its four
 // independent B operand sets need the compiler's default accumulator placement, while the Gram
 // kernel is built with MFMA results in VGPRs).
 template <typename T, int KB, int PREC>
 hipError_t launch_linear_pairwise_fused(const GramFusedArgs<T> &a, const double *sxd,
-                                        const double *syd, size_t shmem, hipStream_t st);
+                                        const double *syd, hipStream_t st);
 
 }  // namespace sigpde

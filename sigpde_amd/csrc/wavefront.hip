@@ -417,12 +417,7 @@ static hipError_t launch16(int order, int N, int64_t waves, bool global, hipStre
 
 // Workgroups of `kernel` (64 W threads, lds bytes of LDS) the current GPU holds at once.
 static int64_t resident_groups(const void *kernel, int W, size_t lds) {
-  int dev = 0, cus = 0, per = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 64 * W, lds) != hipSuccess)
-    return 0;
-  return (int64_t)cus * per;
+  return (int64_t)cu_count() * blocks_per_cu(kernel, 64 * W, lds);
 }
 
 // Pairs of several bands of 64 refined rows run one pair per workgroup of W waves (mw_solver.h)
