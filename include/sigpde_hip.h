@@ -16,8 +16,9 @@
  *     <= 2^30, in the entry points that take increments (past 17695 SIGPDE_F32 / 14927
  *     SIGPDE_F64 refined columns the band-boundary lines go to global memory, allocated
  *     stream-ordered by the call: hipMallocAsync / hipFreeAsync on `stream`); the fused linear
- *     entry points take up to 9679 refined columns (their lines stay in LDS). Larger inputs
- *     return SIGPDE_EINVAL naming the limit.
+ *     entry points take up to 9679 refined columns (4559 in SIGPDE_PREC_FP64), and keep their
+ *     lines in a stream-ordered global block too when lines that long in LDS would hold fewer
+ *     waves per CU (the same results). Larger inputs return SIGPDE_EINVAL naming the limit.
  *
  * The reference has no C interface: its "FFI" is the Python/Numba launch layer
  * sigpde/PDESolvers.py, whose kernel launches these entry points replace one for one.
