@@ -17,8 +17,8 @@
  *     SIGPDE_F64 refined columns the band-boundary lines go to global memory, allocated
  *     stream-ordered by the call: hipMallocAsync / hipFreeAsync on `stream`); the fused linear
  *     entry points take up to 9679 refined columns (4559 in SIGPDE_PREC_FP64), and keep their
- *     lines in a stream-ordered global block too when lines that long in LDS would hold fewer
- *     waves per CU (the same results). Larger inputs return SIGPDE_EINVAL naming the limit.
+ *     lines in a stream-ordered global block too when lines that long in LDS would hold far
+ *     fewer waves per CU (the same results). Larger inputs return SIGPDE_EINVAL naming the limit.
  *
  * The reference has no C interface: its "FFI" is the Python/Numba launch layer
  * sigpde/PDESolvers.py, whose kernel launches these entry points replace one for one.

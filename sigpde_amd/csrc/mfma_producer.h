@@ -285,11 +285,13 @@ struct FusedArith {
       PREC == kPrecFp32 ? Arith::Ref32 : (PREC == kPrecFp64 ? Arith::Ref64 : Arith::Mixed);
 };
 
-// Where a fused launch keeps its band-boundary lines: in LDS (r16_lds_bytes) unless lines that
-// long would let fewer waves share a CU than the kernel's registers allow (long rows of
-// columns), then in global memory, one block of r16_line_bytes per wave of the launch (at most
-// kFusedLineBytes in all), with only the ring in LDS. Results are the same either way.
-// k_lds / k_gl: the kernel of each placement. Returns the launch's LDS bytes; sets gl.
+// Where a fused launch keeps its band-boundary lines: in LDS (r16_lds_bytes) unless global
+// memory lets at least 1.5 times as many waves share a CU (long rows of columns, whose lines fill
+// LDS), then one block of r16_line_bytes per wave of the launch (at most kFusedLineBytes in all),
+// with only the ring in LDS. A smaller gain does not pay for the global line accesses (measured:
+// 12 instead of 9 waves per CU was 12% slower, 12 instead of 7 was 7% faster). Results are the
+// same either way. k_lds / k_gl: the kernel of each placement. Returns the launch's LDS bytes;
+// sets gl.
 constexpr size_t kFusedLineBytes = (size_t)4 << 30;
 template <Arith A>
 inline size_t fused_line_placement(const void *k_lds, const void *k_gl, int N, int64_t waves,
@@ -298,7 +300,8 @@ inline size_t fused_line_placement(const void *k_lds, const void *k_gl, int N, i
   gl = false;
   if ((double)waves * (double)r16_line_bytes<A>(N) > (double)kFusedLineBytes) return lds_lines;
   const int occ_lds = lds_lines <= kMaxLdsBytes ? blocks_per_cu(k_lds, 64, lds_lines) : 0;
-  gl = blocks_per_cu(k_gl, 64, lds_ring) > occ_lds;
+  const int occ_gl = blocks_per_cu(k_gl, 64, lds_ring);
+  gl = occ_gl > occ_lds && 2 * occ_gl >= 3 * occ_lds;
   return gl ? lds_ring : lds_lines;
 }
 
