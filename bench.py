@@ -417,9 +417,7 @@ def serving_leg(args, dev):
     `sweep` repeats the Poisson run at 50% and 90% of the saturated rate."""
     from sigpde_amd import serving as S
 
-    warm = S.synthetic_requests(6, seed=99, device=dev, precision="fp16x2", max_batch_size=16,
-                                lengths=(16, 64))
-    S.Server(dev, streams=4).run(warm, realtime=False)
+    S.Server(dev, streams=4).warmup(precisions=("ref", "fp16x2"))
     torch.cuda.synchronize()
 
     def serve(rate):

@@ -386,6 +386,31 @@ class Server:
     def pending(self) -> int:
         return sum(len(v) for v in self._queue.values()) + len(self._inflight)
 
+    def warmup(self, precisions=("ref", "fp16x2"), dtypes=(torch.float32,)) -> None:
+        """Serves one tiny request of every kind in each precision and dtype, then forgets them.
+        The runtime loads the library's device code at the first launch of a kernel from each of
+        its compiled units (0.1-0.3 s each); warming up at start keeps that, and the allocator's
+        first blocks, off the first real requests' latency."""
+        g = torch.Generator().manual_seed(0)
+        reqs = []
+        for dt in dtypes:
+            x = (torch.randn(4, 9, 4, generator=g, dtype=torch.float64) * 0.3).cumsum(1)
+            x = x.to(device=self.device, dtype=dt)
+            for prec in precisions:
+                for kind in KINDS:
+                    if kind == "normalization" and prec != "ref":
+                        continue
+                    y = None if kind in ("gram", "robust_gram") else x
+                    reqs.append(Request(kind, x, y, dyadic_order=1, precision=prec))
+        launches, done = self.launches, len(self.completed)
+        self.run(reqs, realtime=False)
+        torch.cuda.synchronize(self.device)
+        failed = [r for r in reqs if r.error is not None]
+        if failed:
+            raise RuntimeError(f"warmup request failed: {failed[0].error!r}")
+        self.launches = launches
+        del self.completed[done:]
+
     def run(self, requests: List[Request], realtime: bool = True) -> List[Request]:
         """Serves `requests` (sorted by arrival). realtime: submit each at its arrival time
         (seconds after start); otherwise all at once."""
@@ -499,6 +524,7 @@ def main(argv=None):
     torch.cuda.synchronize()
     srv = Server(dev, hbm_budget=int(args.budget_gb * (1 << 30)), streams=args.streams,
                  use_graphs=args.graphs)
+    srv.warmup(precisions=tuple(sorted({"ref", args.precision})))
     t0 = time.perf_counter()
     srv.run(reqs, realtime=args.rate < math.inf)
     torch.cuda.synchronize()

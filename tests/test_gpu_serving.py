@@ -160,3 +160,14 @@ def test_summary_has_latency_percentiles():
     s = S.summarize(reqs, time.perf_counter() - t0)
     assert s["failed"] == 0 and s["requests"] == 12
     assert 0 < s["latency_ms"]["p50"] <= s["latency_ms"]["p99"]
+
+
+def test_warmup_serves_every_kind_and_forgets_it():
+    """Server.warmup runs a tiny request of every kind (loading the device code) and leaves no
+    trace in the server's counters; real requests are served after it as usual."""
+    srv = S.Server(DEV, streams=2)
+    srv.warmup(precisions=("ref", "fp16x2", "fp64"), dtypes=(torch.float32, torch.float64))
+    assert srv.launches == 0 and srv.completed == [] and not srv.pending()
+    reqs = S.synthetic_requests(4, seed=3, device=DEV, max_batch_size=4, lengths=(16, 32))
+    srv.run(reqs, realtime=False)
+    assert all(r.error is None for r in reqs) and len(srv.completed) == 4
