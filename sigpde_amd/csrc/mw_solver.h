@@ -2,9 +2,10 @@
 //
 // Same scheme and cell arithmetic as ring_solver.h / ring16.h (stojl/SigPDE
 // sigpde/cuda_device_functions.py:3-5, sigpde/cuda_pairwise_kernels.py:8-39): lane rho of a wave
-// owns refined row i = 64 b + rho + 1 of band b and handles column v = tau - rho + 1 at band step
-// tau; the value above arrives from lane rho - 1 by DPP (wave_shr:1), and the same coefs / cell
-// functions round every cell as the other solvers do, so a pair's result is the same bit for bit.
+// owns refined rows 64 R b + R rho + 1 .. + R of band b (R = 1, or 2 from dyadic order 1: rows
+// that share one row of increments) and handles column v = tau - rho + 1 at band step tau; the
+// value above arrives from lane rho - 1 by DPP (wave_shr:1), and the same coefs / cell functions
+// round every cell as the other solvers do, so a pair's result is the same bit for bit.
 //
 // What differs is that the bands of one pair run concurrently on the W waves, pipelined:
 //   * Band b runs on wave b mod W during periods (16 steps each) [b D, b D + P), P = ceil((N+63)/16)
@@ -22,7 +23,8 @@
 //   * Increments are read straight from global memory (cached: a path's increments are reused by
 //     every root-find iteration), a period ahead of their use: a run of 16 columns of the lane's
 //     row per period.
-// Sequential steps per pair: ((nb - 1) D + P) 16 instead of nb P 16 on one wave (nb = ceil(M/64)).
+// Sequential steps per pair: ((nb - 1) D + P) 16 instead of nb P 16 on one wave
+// (nb = ceil(M / (64 R))).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -38,15 +40,18 @@ namespace sigpde {
 struct MwGeom {
   int m, n, order;  // original increments (rows / columns) and dyadic order
   int M, N;         // refined rows / columns
-  int nb;           // bands of 64 rows
+  int R;            // refined rows per lane (a power of two dividing 2^order: they share a row
+                    // of increments)
+  int nb;           // bands of 64 R rows
   int P;            // periods per band
   int D;            // periods between band starts
   int win;          // window entries
   int T;            // periods of the whole pair
-  __host__ __device__ MwGeom(int m_, int n_, int o, int W) : m(m_), n(n_), order(o) {
+  __host__ __device__ MwGeom(int m_, int n_, int o, int W, int R_ = 1)
+      : m(m_), n(n_), order(o), R(R_) {
     M = m << o;
     N = n << o;
-    nb = (M + 63) / 64;
+    nb = (M + 64 * R - 1) / (64 * R);
     P = (N + 63 + 15) / 16;
     const int need = (P + 1 + (W - 2)) / (W - 1);  // ceil((P + 1) / (W - 1))
     D = need > 5 ? need : 5;
@@ -102,8 +107,10 @@ __host__ __device__ inline size_t mw_lds_bytes(const MwGeom &g, int W) {
 
 // Solves one pair: increments inc (m x n, row stride ld), pair scale `scale` applied to each
 // increment in RT when `scaled` (as the other producers do), Scaled* modes' per-call factor s.
-// Every thread of the workgroup (64 W threads) must call it; all get K[M][N].
-template <Arith A, int W>
+// R refined rows per lane (g.R == R): lane rho of band b holds rows 64 R b + R rho + 1 .. + R,
+// which share one row of increments, so the coefficients and the shift from the lane above serve
+// R cells a step. Every thread of the workgroup (64 W threads) must call it; all get K[M][N].
+template <Arith A, int W, int R>
 __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, int64_t ld,
                            typename ArithTraits<A>::RT scale, bool scaled, const MwGeom &g,
                            double s, void *lds) {
@@ -118,7 +125,9 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
   BT *outWin = wins + ((wave + 1) % W) * g.win;       // this wave's band b -> band b+1
   const int jo_max = g.n - 1;
 
-  double k = 1.0, kPrev = 1.0, result = 0.0;
+  double k[R], kPrev = 1.0, result = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) k[r] = 1.0;
   // k of lane rho - 1 (lane 0: `top`, the band above). In the float32 modes every cell value is
   // a float (cell rounds it), so the float is shifted: one DPP move instead of two, no conversion
   // of the top value, the same value.
@@ -130,13 +139,25 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
       return shift_down<64>(kk, (double)top);
     }
   };
+  // one step of the lane's R rows at its current column: the row above the first is `up`
+  // (lane rho - 1's last row), each further row's the one computed just before it
+  auto step_rows = [&](double up, double cA, double cB) {
+    double above = up, aboveOld = kPrev;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double old = k[r];
+      k[r] = cell<A>(above, old, aboveOld, cA, cB);
+      aboveOld = old;
+      above = k[r];
+    }
+  };
   RT cc[16], cn[16];
   // increments of period q of band b, for this lane: columns v = 16 q + st - lane + 1. When all
   // 16 lie inside 1 .. N (16 q >= 63, 16 q + 16 <= N) they are one run of the lane's row: read as
   // vector loads at order 0 and as the few distinct values at higher orders, so that a wave's
   // load instructions touch each cache line a few times rather than once per column.
   auto fetch = [&](RT (&dst)[16], int b, int q) {
-    int io = ((64 * b + lane) >> g.order);
+    int io = ((64 * R * b + R * lane) >> g.order);
     io = io < g.m ? io : g.m - 1;
     const RT *row = inc + (int64_t)io * ld;
     const int c0 = 16 * q - lane;  // v - 1 of step 0
@@ -186,15 +207,16 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
     }
     if (b >= 0) {
       if (q == 0) {
-        k = 1.0;      // k[i][0]
-        kPrev = 1.0;  // k[i-1][0]
+#pragma unroll
+        for (int r = 0; r < R; ++r) k[r] = 1.0;  // k[i][0]
+        kPrev = 1.0;                              // k[i-1][0]
         fetch(cc, b, 0);
       } else {
 #pragma unroll
         for (int st = 0; st < 16; ++st) cc[st] = cn[st];
       }
       if (q + 1 < g.P) fetch(cn, b, q + 1);
-      const int i = 64 * b + lane + 1;  // this lane's row (1-based)
+      const int i = 64 * R * b + R * lane + 1;  // this lane's first row (1-based)
       // lane 63 writes its columns 16 q + st - 62 at window slots wb + st + 1 (win is a multiple
       // of 16: only the last may wrap)
       const int wb = (16 * q) % g.win;
@@ -215,20 +237,20 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
       const bool hand = lane == 63 && b + 1 < g.nb;  // this lane feeds the next band
       // Steady periods: every lane's 16 columns lie inside 1 .. N - 1 and every row inside the
       // grid, so no step needs a range test (and none holds the corner, column N).
-      const bool steady = 16 * q >= 63 && 16 * q + 16 < g.N && 64 * b + 64 <= g.M;
+      const bool steady = 16 * q >= 63 && 16 * q + 16 < g.N && 64 * R * (b + 1) <= g.M;
       if (steady) {
 #pragma unroll
         for (int st = 0; st < 16; ++st) {
           if (st % 4 == 0 && st < 12) tops4(tnxt, st / 4 + 1);
-          const double up = shift_up(k, tcur[st % 4]);
+          const double up = shift_up(k[R - 1], tcur[st % 4]);
           if (st % 4 == 3 && st < 15) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) tcur[e] = tnxt[e];
           }
           double cA, cB;
           coefs<A>(cc[st], s, 0.0, 0.0, cA, cB);
-          k = cell<A>(up, k, kPrev, cA, cB);
-          if (hand) outWin[st < 15 ? wb + st + 1 : w15] = (BT)k;
+          step_rows(up, cA, cB);
+          if (hand) outWin[st < 15 ? wb + st + 1 : w15] = (BT)k[R - 1];
           kPrev = up;
         }
       } else {
@@ -236,17 +258,22 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
         for (int st = 0; st < 16; ++st) {
           if (st % 4 == 0 && st < 12) tops4(tnxt, st / 4 + 1);
           const int v = 16 * q + st - lane + 1;  // this lane's column (1-based)
-          const double up = shift_up(k, tcur[st % 4]);
+          const double up = shift_up(k[R - 1], tcur[st % 4]);
           if (st % 4 == 3 && st < 15) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) tcur[e] = tnxt[e];
           }
-          if (v >= 1 && v <= g.N && i <= g.M) {
+          // M is a multiple of R: a lane's rows are all inside the grid or all past it
+          if (v >= 1 && v <= g.N && i + R - 1 <= g.M) {
             double cA, cB;
             coefs<A>(cc[st], s, 0.0, 0.0, cA, cB);
-            k = cell<A>(up, k, kPrev, cA, cB);
-            if (hand) outWin[st < 15 ? wb + st + 1 : w15] = (BT)k;
-            if (i == g.M && v == g.N) result = k;
+            step_rows(up, cA, cB);
+            if (hand) outWin[st < 15 ? wb + st + 1 : w15] = (BT)k[R - 1];
+            if (v == g.N) {
+#pragma unroll
+              for (int r = 0; r < R; ++r)
+                if (i + r == g.M) result = k[r];
+            }
           }
           kPrev = up;
         }
@@ -255,7 +282,7 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
     __syncthreads();
   }
   // the corner lane publishes K[M][N]
-  if (64 * (g.nb - 1) + lane + 1 == g.M && wave == (g.nb - 1) % W) *res = result;
+  if ((g.M - 1 - 64 * R * (g.nb - 1)) / R == lane && wave == (g.nb - 1) % W) *res = result;
   __syncthreads();
   const double r = *res;
   __syncthreads();  // the next call may overwrite the slot

@@ -102,16 +102,16 @@ __global__ __launch_bounds__(kWave) void k_pde_pairwise(PairwiseArgs<T> a, int64
   });
 }
 
-// One pair per workgroup of W waves (mw_solver.h), for few long pairs.
-template <typename T, int W>
+// One pair per workgroup of W waves (mw_solver.h), for few long pairs; R refined rows per lane.
+template <typename T, int W, int R>
 __global__ __launch_bounds__(64 * W) void k_pde_pairwise_mw(PairwiseArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int64_t b = blockIdx.x;
-  const MwGeom g(a.m, a.n, a.order, W);
+  const MwGeom g(a.m, a.n, a.order, W, R);
   const bool scaled = a.sx || a.sy;
   const T scale = scaled ? (a.sy ? a.sy[b] : (T)1) * (a.sx ? a.sx[b] : (T)1) : (T)1;
-  const double r = mw_solve<RefArith<T>::kPlain, W>(a.inc + b * a.stride_b, a.n, scale, scaled,
-                                                    g, 1.0, smem_raw);
+  const double r = mw_solve<RefArith<T>::kPlain, W, R>(a.inc + b * a.stride_b, a.n, scale,
+                                                       scaled, g, 1.0, smem_raw);
   if (threadIdx.x == 0) a.out[b] = (T)r;
 }
 
@@ -327,16 +327,16 @@ __global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a, int64_t 
 }
 
 // The same root-find with one path per workgroup of W waves (mw_solver.h: the bands of each
-// wavefront evaluation run concurrently), for few long paths.
-template <typename T, int W>
+// wavefront evaluation run concurrently), for few long paths; R refined rows per lane.
+template <typename T, int W, int R>
 __global__ __launch_bounds__(64 * W) void k_robust_norm_mw(RobustArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int64_t b = blockIdx.x;
-  const MwGeom g(a.m, a.m, a.order, W);
+  const MwGeom g(a.m, a.m, a.order, W, R);
   const T *inc = a.inc + b * a.stride_b;
   T K1 = (T)1, Tt = (T)1;
   if (a.normalizer == 0) {
-    K1 = (T)mw_solve<RefArith<T>::kPlain, W>(inc, a.m, (T)1, false, g, 1.0, smem_raw);
+    K1 = (T)mw_solve<RefArith<T>::kPlain, W, R>(inc, a.m, (T)1, false, g, 1.0, smem_raw);
     if (isnan(K1)) K1 = (T)INFINITY;
     Tt = log_normalizer<T>(K1);
   } else {
@@ -348,8 +348,8 @@ __global__ __launch_bounds__(64 * W) void k_robust_norm_mw(RobustArgs<T> a) {
   bool active = a.maxit > 0;  // uniform over the workgroup: one path
   while (active) {
     const double xt = rf.next();
-    const double sol = mw_solve<RefArith<T>::kScaled, W>(inc, a.m, (T)1, false, g, xt * xt,
-                                                        smem_raw);
+    const double sol = mw_solve<RefArith<T>::kScaled, W, R>(inc, a.m, (T)1, false, g, xt * xt,
+                                                           smem_raw);
     if (rf.update(xt, sol, a.tol, a.maxit)) active = false;
   }
   if (threadIdx.x == 0) {
@@ -428,12 +428,12 @@ static int64_t resident_groups(const void *kernel, int W, size_t lds) {
 // for float64, whose registers would spill at 16 waves per CU). Sets W and returns the
 // workgroup's LDS bytes, or returns 0 when the pairs take the other routes.
 template <typename T>
-static size_t mw_route(int64_t pairs, int m, int n, int order, const void *kern16,
+static size_t mw_route(int64_t pairs, int m, int n, int order, int R, const void *kern16,
                        const void *kern8, int &W) {
   for (const int w : {16, 8}) {
     const void *kern = w == 16 ? kern16 : kern8;
     if (!kern) continue;
-    const MwGeom g(m, n, order, w);
+    const MwGeom g(m, n, order, w, R);
     if (g.nb < w / 2 || g.D >= g.P) continue;  // D >= P: bands would not overlap
     const size_t lds = mw_lds_bytes<RefArith<T>::kPlain>(g, w);
     if (lds > kMaxLdsBytes || pairs > resident_groups(kern, w, lds)) continue;
@@ -443,23 +443,56 @@ static size_t mw_route(int64_t pairs, int m, int n, int order, const void *kern1
   return 0;
 }
 
+// The multi-wave route with R refined rows per lane. From dyadic order 1, R = 2 is tried first:
+// two rows share each row of increments, so the coefficients and the shift from the lane above
+// serve two cells (root-finds 20-35% faster); its extra registers can hold fewer workgroups per
+// CU, and pairs past that count try R = 1 before the one-wave routes. Launches and returns true
+// when mw_route takes the pairs (e: the launch status).
+template <typename T, int R>
+static bool mw_pairwise(const PairwiseArgs<T> &a, hipStream_t st, hipError_t &e) {
+  int W = 0;
+  const void *k16 = nullptr;
+  if constexpr (sizeof(T) == 4) k16 = reinterpret_cast<const void *>(&k_pde_pairwise_mw<T, 16, R>);
+  const void *k8 = reinterpret_cast<const void *>(&k_pde_pairwise_mw<T, 8, R>);
+  const size_t lds = mw_route<T>(a.B, a.m, a.n, a.order, R, k16, k8, W);
+  if (!lds) return false;
+  if constexpr (sizeof(T) == 4) {
+    if (W == 16) {
+      k_pde_pairwise_mw<T, 16, R><<<dim3((unsigned)a.B), dim3(64 * 16), lds, st>>>(a);
+      e = hipGetLastError();
+      return true;
+    }
+  }
+  k_pde_pairwise_mw<T, 8, R><<<dim3((unsigned)a.B), dim3(64 * 8), lds, st>>>(a);
+  e = hipGetLastError();
+  return true;
+}
+
+template <typename T, int R>
+static bool mw_robust(const RobustArgs<T> &a, hipStream_t st, hipError_t &e) {
+  int W = 0;
+  const void *k16 = nullptr;
+  if constexpr (sizeof(T) == 4) k16 = reinterpret_cast<const void *>(&k_robust_norm_mw<T, 16, R>);
+  const void *k8 = reinterpret_cast<const void *>(&k_robust_norm_mw<T, 8, R>);
+  const size_t lds = mw_route<T>(a.B, a.m, a.m, a.order, R, k16, k8, W);
+  if (!lds) return false;
+  if constexpr (sizeof(T) == 4) {
+    if (W == 16) {
+      k_robust_norm_mw<T, 16, R><<<dim3((unsigned)a.B), dim3(64 * 16), lds, st>>>(a);
+      e = hipGetLastError();
+      return true;
+    }
+  }
+  k_robust_norm_mw<T, 8, R><<<dim3((unsigned)a.B), dim3(64 * 8), lds, st>>>(a);
+  e = hipGetLastError();
+  return true;
+}
+
 template <typename T>
 static hipError_t launch_pairwise(const PairwiseArgs<T> &a, hipStream_t st) {
   constexpr Arith A = RefArith<T>::kPlain;
-  int W = 0;
-  const void *k16 = nullptr;
-  if constexpr (sizeof(T) == 4) k16 = reinterpret_cast<const void *>(&k_pde_pairwise_mw<T, 16>);
-  const void *k8 = reinterpret_cast<const void *>(&k_pde_pairwise_mw<T, 8>);
-  if (const size_t lds = mw_route<T>(a.B, a.m, a.n, a.order, k16, k8, W)) {
-    if constexpr (sizeof(T) == 4) {
-      if (W == 16) {
-        k_pde_pairwise_mw<T, 16><<<dim3((unsigned)a.B), dim3(64 * 16), lds, st>>>(a);
-        return hipGetLastError();
-      }
-    }
-    k_pde_pairwise_mw<T, 8><<<dim3((unsigned)a.B), dim3(64 * 8), lds, st>>>(a);
-    return hipGetLastError();
-  }
+  hipError_t e = hipSuccess;
+  if ((a.order >= 1 && mw_pairwise<T, 2>(a, st, e)) || mw_pairwise<T, 1>(a, st, e)) return e;
   const int N = a.n << a.order;
   const bool f16 = g16_fits<T>(N), f64 = g64_fits<T>(N);
   if ((a.B >= kManyPairs && f16) || !f64) {
@@ -498,20 +531,8 @@ static hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st) {
 template <typename T>
 static hipError_t launch_robust(const RobustArgs<T> &a, hipStream_t st) {
   constexpr Arith A = RefArith<T>::kPlain;
-  int W = 0;
-  const void *k16 = nullptr;
-  if constexpr (sizeof(T) == 4) k16 = reinterpret_cast<const void *>(&k_robust_norm_mw<T, 16>);
-  const void *k8 = reinterpret_cast<const void *>(&k_robust_norm_mw<T, 8>);
-  if (const size_t lds = mw_route<T>(a.B, a.m, a.m, a.order, k16, k8, W)) {
-    if constexpr (sizeof(T) == 4) {
-      if (W == 16) {
-        k_robust_norm_mw<T, 16><<<dim3((unsigned)a.B), dim3(64 * 16), lds, st>>>(a);
-        return hipGetLastError();
-      }
-    }
-    k_robust_norm_mw<T, 8><<<dim3((unsigned)a.B), dim3(64 * 8), lds, st>>>(a);
-    return hipGetLastError();
-  }
+  hipError_t e = hipSuccess;
+  if ((a.order >= 1 && mw_robust<T, 2>(a, st, e)) || mw_robust<T, 1>(a, st, e)) return e;
   const int N = a.m << a.order;
   const bool f16 = g16_fits<T>(N), f64 = g64_fits<T>(N);
   if ((a.B >= kManyPairs && f16) || !f64) {

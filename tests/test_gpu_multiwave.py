@@ -1,6 +1,7 @@
 """GPU parity of the multi-wave route (sigpde_amd/csrc/mw_solver.h): pairs of at least 4 bands of
 64 refined rows run one pair per workgroup of 8 waves (16 in float32 from 8 bands), the bands of each
-pair pipelined across the waves, when the GPU holds all of the pairs' workgroups at once. Checked
+pair pipelined across the waves, when the GPU holds all of the pairs' workgroups at once; from
+dyadic order 1 a lane holds two refined rows (one row of increments) when that still fits. Checked
 against the oracle, and bit for bit against the one-wave-per-pair route (the same pairs repeated
 past the GPU's resident workgroups): the cell arithmetic is shared, so the route must not change a
 result.
@@ -70,12 +71,13 @@ def test_multiwave_route_equals_one_wave_route_bitwise(dtype):
     assert torch.equal(few, many[:B])
 
 
-@pytest.mark.parametrize("m", [300, 1100, 2100])
-def test_robust_norm_multiwave(m):
-    B, order = 2, 0
+@pytest.mark.parametrize("m,order", [(300, 0), (1100, 0), (2100, 0),
+                                     (300, 1), (150, 2)])  # orders 1-2: two rows per lane
+def test_robust_norm_multiwave(m, order):
+    B = 2
     rng = np.random.default_rng(17)
     dx = rng.normal(size=(B, m, 3)) * (1.3 / np.sqrt(m))
-    inc = np.einsum("bsk,btk->bst", dx, dx)
+    inc = np.einsum("bsk,btk->bst", dx, dx) / 4.0 ** order
     lam_ref, k_ref, it_ref = O.norm_factors(inc, order, tol=1e-10, maxit=100)
     lam = torch.zeros(B, dtype=torch.float64, device=DEV)
     kn = torch.zeros(B, dtype=torch.float64, device=DEV)
@@ -86,7 +88,7 @@ def test_robust_norm_multiwave(m):
     assert max_rel(lam.cpu().numpy(), lam_ref, floor=1e-3) < 1e-9
     assert np.array_equal(iters.cpu().numpy(), it_ref)
     # bit for bit against the one-wave route: more paths than the GPU holds workgroups of the
-    # float64 root-find (174 registers per lane: one workgroup of 8 waves per CU)
+    # float64 root-find (166-178 registers per lane: one workgroup of 8 waves per CU)
     nm = 2 * torch.cuda.get_device_properties(0).multi_processor_count + 1
     many = t.repeat((nm + B - 1) // B, 1, 1)[:nm].contiguous()
     lam2 = torch.zeros(nm, dtype=torch.float64, device=DEV)
