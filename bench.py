@@ -136,13 +136,13 @@ def fused_roofline(args, X, Y, ms):
     cells = pairs * m * n
     flops_wave = cells * FLOPS_PER_CELL
     flops_inc = pairs * 2 * args.dim * (X.shape[1] - 1) * (Y.shape[1] - 1)
-    name = {"fp16x2": "k_linear_gram_fused<float, 2, 1> (fp16x2 MFMA increments + double "
-                      "wavefront, one launch)",
-            "bf16x3": "k_linear_gram_fused<float, 2, 3> (bf16x3 MFMA increments + double "
-                      "wavefront, one launch)",
-            "fp32": "k_linear_gram_fused<float, 2, 2> (fp32 MFMA increments + float32-typed "
-                    "wavefront, one launch)",
-            "fp64": "k_linear_gram_fused<float, 2, 4> (fp64 MFMA increments + double "
+    name = {"fp16x2": "k_linear_gram_fused<float, 2, 1, 1, false> (fp16x2 MFMA increments + "
+                      "double wavefront, one launch)",
+            "bf16x3": "k_linear_gram_fused<float, 2, 3, 1, false> (bf16x3 MFMA increments + "
+                      "double wavefront, one launch)",
+            "fp32": "k_linear_gram_fused<float, 2, 2, 1, false> (fp32 MFMA increments + "
+                    "float32-typed wavefront, one launch)",
+            "fp64": "k_linear_gram_fused<float, 2, 4, 1, false> (fp64 MFMA increments + double "
                     "wavefront, one launch)"}[args.mode]
     traffic = MEASURED_TRAFFIC.get((args.mode, args.paths, args.length, args.dim, args.order))
     alg_bytes = (X.shape[0] * (X.shape[1] - 1) + Y.shape[0] * (Y.shape[1] - 1)) * \
