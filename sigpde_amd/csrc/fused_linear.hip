@@ -336,7 +336,8 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
       bool gl = false;
       const size_t shmem = fused_line_placement<A>(
           reinterpret_cast<const void *>(&k_linear_gram_fused<T, KB, PREC, R, false>),
-          reinterpret_cast<const void *>(&k_linear_gram_fused<T, KB, PREC, R, true>), N, grid, gl);
+          reinterpret_cast<const void *>(&k_linear_gram_fused<T, KB, PREC, R, true>), N, grid, st,
+          gl);
       return with_fused_lines(gl, (size_t)grid * r16_line_bytes<A>(N), st, [&](void *lines) {
         if (lines)
           k_linear_gram_fused<T, KB, PREC, R, true><<<g3, b3, shmem, st>>>(a, sxd, syd, lines);

@@ -66,7 +66,7 @@ hipError_t launch_linear_pairwise_fused(const GramFusedArgs<T> &a, const double 
     const size_t shmem = fused_line_placement<A>(
         reinterpret_cast<const void *>(&k_linear_pairwise_fused<T, KB, PREC, R, false>),
         reinterpret_cast<const void *>(&k_linear_pairwise_fused<T, KB, PREC, R, true>), N, waves,
-        gl);
+        st, gl);
     return with_fused_lines(gl, (size_t)waves * r16_line_bytes<A>(N), st, [&](void *lines) {
       if (lines)
         k_linear_pairwise_fused<T, KB, PREC, R, true><<<grid, block, shmem, st>>>(a, sxd, syd,

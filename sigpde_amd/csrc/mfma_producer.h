@@ -290,14 +290,18 @@ struct FusedArith {
 // LDS), then one block of r16_line_bytes per wave of the launch (at most kFusedLineBytes in all),
 // with only the ring in LDS. A smaller gain does not pay for the global line accesses (measured:
 // 12 instead of 9 waves per CU was 12% slower, 12 instead of 7 was 7% faster). Results are the
-// same either way. k_lds / k_gl: the kernel of each placement. Returns the launch's LDS bytes;
-// sets gl.
+// same either way. A stream under graph capture keeps the lines in LDS (they always fit there
+// below the fused column limit), so that a captured launch allocates nothing. k_lds / k_gl: the
+// kernel of each placement. Returns the launch's LDS bytes; sets gl.
 constexpr size_t kFusedLineBytes = (size_t)4 << 30;
 template <Arith A>
 inline size_t fused_line_placement(const void *k_lds, const void *k_gl, int N, int64_t waves,
-                                   bool &gl) {
+                                   hipStream_t st, bool &gl) {
   const size_t lds_lines = r16_lds_bytes<A>(N), lds_ring = r16_ring_bytes<A>();
   gl = false;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
+    return lds_lines;
   if ((double)waves * (double)r16_line_bytes<A>(N) > (double)kFusedLineBytes) return lds_lines;
   const int occ_lds = lds_lines <= kMaxLdsBytes ? blocks_per_cu(k_lds, 64, lds_lines) : 0;
   const int occ_gl = blocks_per_cu(k_gl, 64, lds_ring);

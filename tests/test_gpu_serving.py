@@ -149,6 +149,26 @@ def test_graph_replays_equal_eager(prec):
     assert len(srv._graphs) == 1  # one shape class, one padded size: captured once
 
 
+def test_graph_replays_equal_eager_long_rows_of_columns():
+    """2000 refined columns: an eager fused launch keeps its boundary lines in global memory
+    (allocated stream-ordered), a captured one keeps them in LDS so that the graph allocates
+    nothing; the results are the same bit for bit."""
+    k = sig.kernels.LinearKernel()
+    srv = S.Server(DEV, use_graphs=True, streams=1)
+    for rep in range(3):
+        # scaled by 1/8 (exact): K of 1000-step paths stays moderate
+        reqs = [S.Request("pairwise", quantized(2, 1001, 4, 40 + 10 * rep + i) / 8,
+                          quantized(2, 1001, 4, 45 + 10 * rep + i) / 8, dyadic_order=1,
+                          precision="fp16x2") for i in range(2)]
+        srv.run(reqs, realtime=False)
+        torch.cuda.synchronize()
+        eager = [sig.SigPDE(k, 1, "fp16x2").pairwise(r.x, r.y) for r in reqs]
+        torch.cuda.synchronize()
+        for r, e in zip(reqs, eager):
+            assert r.error is None, r.error
+            assert torch.equal(r.result, e), rep
+
+
 def test_summary_has_latency_percentiles():
     reqs = S.synthetic_requests(12, seed=5, device=DEV, max_batch_size=8, lengths=(16, 40),
                                 orders=(0, 1))
