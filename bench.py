@@ -420,15 +420,27 @@ def serving_leg(args, dev):
     S.Server(dev, streams=4).warmup(precisions=("ref", "fp16x2"))
     torch.cuda.synchronize()
 
+    # one request set (shapes and paths from seed 7) for every run; each rate draws its own
+    # Poisson arrival times (seed 8)
+    base = S.synthetic_requests(args.serving, seed=7, device=dev, precision="fp16x2")
+    torch.cuda.synchronize()
+
     def serve(rate):
-        reqs = S.synthetic_requests(args.serving, seed=7, rate=rate, device=dev,
-                                    precision="fp16x2")
-        torch.cuda.synchronize()
+        if np.isfinite(rate):
+            arrivals = np.cumsum(np.random.default_rng(8).exponential(1.0 / rate, len(base)))
+        else:
+            arrivals = np.zeros(len(base))
+        reqs = [S.Request(r.kind, r.x, r.y, r.static_kernel, r.dyadic_order, r.precision,
+                          r.normalizer, r.tol, r.maxit, arrival=float(a))
+                for r, a in zip(base, arrivals)]
         srv = S.Server(dev, hbm_budget=64 << 30, streams=4)
         t0 = time.perf_counter()
         srv.run(reqs, realtime=True)
         torch.cuda.synchronize()
-        return S.summarize(reqs, time.perf_counter() - t0), srv.launches
+        out = S.summarize(reqs, time.perf_counter() - t0)
+        print(f"serving leg: {len(reqs)} requests at rate {rate:.3g}/s served in "
+              f"{out['wall_s']:.1f} s", file=sys.stderr, flush=True)
+        return out, srv.launches
 
     sat, _ = serve(float("inf"))
     rate = args.serving_rate if args.serving_rate else 0.7 * sat["requests_per_s"]

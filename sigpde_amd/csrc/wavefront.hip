@@ -327,9 +327,12 @@ __global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a, int64_t 
 }
 
 // The same root-find with one path per workgroup of W waves (mw_solver.h: the bands of each
-// wavefront evaluation run concurrently), for few long paths; R refined rows per lane.
+// wavefront evaluation run concurrently), for few long paths; R refined rows per lane. The
+// float32 kernel of 8 waves and two rows per lane is held to four waves per SIMD (two workgroups
+// per CU: up to 512 paths at once; the compiler's own choice, 130 registers, held one).
 template <typename T, int W, int R>
-__global__ __launch_bounds__(64 * W) void k_robust_norm_mw(RobustArgs<T> a) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(
+    sizeof(T) == 4 && W == 8 && R == 2 ? 4 : 1))) void k_robust_norm_mw(RobustArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int64_t b = blockIdx.x;
   const MwGeom g(a.m, a.m, a.order, W, R);
