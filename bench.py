@@ -84,7 +84,7 @@ def parse():
                    help="skip the all-entries comparison with the float64 GPU path")
     p.add_argument("--no-workloads", action="store_true",
                    help="skip the smaller API workloads (pairwise, Gram, robust) beside the line")
-    p.add_argument("--serving", type=int, default=80, metavar="N",
+    p.add_argument("--serving", type=int, default=300, metavar="N",
                    help="requests of the serving leg (0: skip it)")
     p.add_argument("--serving-rate", type=float, default=None,
                    help="Poisson arrival rate of the serving leg, requests per second (default: "

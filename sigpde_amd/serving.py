@@ -163,6 +163,7 @@ class _Batch:
         self.key = key
         self.reqs = reqs
         self.nbytes = 0
+        self.cost = sum(request_cost(r) for r in reqs)
         self.event: Optional[torch.cuda.Event] = None
         self.outputs: List[torch.Tensor] = []
         self.stream = None
@@ -278,8 +279,16 @@ class Server:
     def _launch(self, b: _Batch) -> None:
         if not self.streams:
             self.streams = [torch.cuda.Stream(device=self.device) for _ in range(self.nstreams)]
-        stream = self.streams[self._next_stream]
-        self._next_stream = (self._next_stream + 1) % len(self.streams)
+        # the stream with the least estimated work in flight (an idle one first), so that a small
+        # request does not queue behind a large one while another stream has little to do; ties
+        # go round-robin
+        load = {id(st): 0.0 for st in self.streams}
+        for f in self._inflight:
+            load[id(f.stream)] += f.cost
+        order = [(self._next_stream + i) % len(self.streams) for i in range(len(self.streams))]
+        pick = min(order, key=lambda i: load[id(self.streams[i])])
+        stream = self.streams[pick]
+        self._next_stream = (pick + 1) % len(self.streams)
         b.stream = stream
         stream.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(stream):
@@ -487,6 +496,22 @@ def request_cells(r: Request) -> int:
         return r.x.shape[0] * y.shape[0] * R * R * m * (y.shape[1] - 1)
     y = r.x if r.y is None else r.y
     return r.x.shape[0] * R * R * m * (y.shape[1] - 1)
+
+
+def request_cost(r: Request) -> float:
+    """Estimated device work of a request, in wavefront cells: its solves, plus for the robust
+    kinds the root-find's evaluations of every path with itself (maxit of them for float32
+    inputs, whose values never reach a tolerance below float resolution; about a dozen for
+    float64)."""
+    cost = float(request_cells(r))
+    if r.kind in ("normalization", "robust_pairwise", "robust_gram"):
+        R = 1 << r.dyadic_order
+        iters = r.maxit if r.x.dtype == torch.float32 else min(r.maxit, 12)
+        sides = [r.x] if r.y is None or r.kind == "normalization" else [r.x, r.y]
+        for p in sides:
+            m = (p.shape[1] - 1) * R
+            cost += (iters + 1) * p.shape[0] * m * m
+    return cost
 
 
 def summarize(reqs: List[Request], wall: float) -> dict:
