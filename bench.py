@@ -28,8 +28,9 @@ Besides the JSON contract fields the line carries:
   serving        the continuous-batching request loop (sigpde_amd.serving) on a bounded sample of
                  the request mix (70% SigPDE / 30% RobustSigPDE, pairwise or Gram, batch 1-256,
                  length 16-512, dim 4/16/64, dyadic order 0-3, float32 Brownian paths, 64 GB HBM
-                 budget): saturated throughput (all queued at once), then Poisson arrivals at 70%
-                 of it: p50 / p99 request latency beside pairs/sec (sweep: at 50% and 90%)
+                 budget; 300 requests by default): saturated throughput (all queued at once),
+                 then the same requests with Poisson arrivals at 70% of it: p50 / p99 request
+                 latency beside pairs/sec (sweep: at 50% and 90%)
 """
 from __future__ import annotations
 
