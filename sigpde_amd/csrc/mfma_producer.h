@@ -285,14 +285,15 @@ struct FusedArith {
       PREC == kPrecFp32 ? Arith::Ref32 : (PREC == kPrecFp64 ? Arith::Ref64 : Arith::Mixed);
 };
 
-// Where a fused launch keeps its band-boundary lines: in LDS (r16_lds_bytes) unless global
-// memory lets at least 1.5 times as many waves share a CU (long rows of columns, whose lines fill
-// LDS), then one block of r16_line_bytes per wave of the launch (at most kFusedLineBytes in all),
-// with only the ring in LDS. A smaller gain does not pay for the global line accesses (measured:
-// 12 instead of 9 waves per CU was 12% slower, 12 instead of 7 was 7% faster). Results are the
-// same either way. A stream under graph capture keeps the lines in LDS (they always fit there
-// below the fused column limit), so that a captured launch allocates nothing. k_lds / k_gl: the
-// kernel of each placement. Returns the launch's LDS bytes; sets gl.
+// Where a fused launch keeps its band-boundary lines: in LDS (r16_lds_bytes) unless the launch
+// has more waves than the GPU holds with LDS lines and global memory lets at least 1.5 times as
+// many share a CU (long rows of columns, whose lines fill LDS), then one block of r16_line_bytes
+// per wave of the launch (at most kFusedLineBytes in all), with only the ring in LDS. A smaller
+// gain, or a launch resident at once either way, does not pay for the global line accesses
+// (measured: 12 instead of 9 waves per CU was 12% slower, 12 instead of 7 was 7% faster).
+// Results are the same either way. A stream under graph capture keeps the lines in LDS (they
+// always fit there below the fused column limit), so that a captured launch allocates nothing.
+// k_lds / k_gl: the kernel of each placement. Returns the launch's LDS bytes; sets gl.
 constexpr size_t kFusedLineBytes = (size_t)4 << 30;
 template <Arith A>
 inline size_t fused_line_placement(const void *k_lds, const void *k_gl, int N, int64_t waves,
@@ -300,12 +301,15 @@ inline size_t fused_line_placement(const void *k_lds, const void *k_gl, int N, i
   const size_t lds_lines = r16_lds_bytes<A>(N), lds_ring = r16_ring_bytes<A>();
   gl = false;
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
+  if (hipStreamIsCapturing(st, &cap) != hipSuccess) {
+    (void)hipGetLastError();  // a failed query only keeps the lines in LDS
     return lds_lines;
+  }
+  if (cap != hipStreamCaptureStatusNone) return lds_lines;
   if ((double)waves * (double)r16_line_bytes<A>(N) > (double)kFusedLineBytes) return lds_lines;
   const int occ_lds = lds_lines <= kMaxLdsBytes ? blocks_per_cu(k_lds, 64, lds_lines) : 0;
   const int occ_gl = blocks_per_cu(k_gl, 64, lds_ring);
-  gl = occ_gl > occ_lds && 2 * occ_gl >= 3 * occ_lds;
+  gl = waves > (int64_t)cu_count() * occ_lds && occ_gl > occ_lds && 2 * occ_gl >= 3 * occ_lds;
   return gl ? lds_ring : lds_lines;
 }
 
