@@ -181,31 +181,42 @@ struct MfmaProducer {
     for (int g = 0; g < NP; ++g) {
       const int a = xs(g), q = ys(g);
       acc_t c = {0, 0, 0, 0};
+      // Products of one magnitude per accumulator, joined by float adds: adding products onto
+      // an accumulator of much smaller magnitude rounds the sum downwards on this MFMA (about
+      // -1 ulp on average, measured; from zero, or onto a sum of like magnitude, the rounding is
+      // unbiased), and that bias, the same sign in every cell, would grow with the path length.
       if constexpr (kF16) {
+        acc_t x = {0, 0, 0, 0};  // lo*hi + hi*lo (2^-11)
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) {
-          c = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al[a][kb], Bh[q][kb], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[a][kb], Bl[q][kb], c, 0, 0, 0);
+          x = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al[a][kb], Bh[q][kb], x, 0, 0, 0);
+          x = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[a][kb], Bl[q][kb], x, 0, 0, 0);
         }
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
           c = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[a][kb], Bh[q][kb], c, 0, 0, 0);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) c[e] += x[e];
       } else if constexpr (kB3) {
-        // hi*lo + mid*mid + lo*hi (2^-16), then hi*mid + mid*hi (2^-8), then hi*hi
+        acc_t x = {0, 0, 0, 0}, y = {0, 0, 0, 0};
+        // hi*lo + mid*mid + lo*hi (2^-16)
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) {
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[a][kb], Bl[q][kb], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am[a][kb], Bm[q][kb], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[a][kb], Bh[q][kb], c, 0, 0, 0);
+          x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[a][kb], Bl[q][kb], x, 0, 0, 0);
+          x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am[a][kb], Bm[q][kb], x, 0, 0, 0);
+          x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[a][kb], Bh[q][kb], x, 0, 0, 0);
         }
+        // hi*mid + mid*hi (2^-8)
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) {
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[a][kb], Bm[q][kb], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am[a][kb], Bh[q][kb], c, 0, 0, 0);
+          y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[a][kb], Bm[q][kb], y, 0, 0, 0);
+          y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am[a][kb], Bh[q][kb], y, 0, 0, 0);
         }
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
           c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[a][kb], Bh[q][kb], c, 0, 0, 0);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) c[e] += y[e] + x[e];
       } else if constexpr (kF64) {
 #pragma unroll
         for (int v = 0; v < NV; ++v)
