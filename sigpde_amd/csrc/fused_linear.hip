@@ -9,7 +9,7 @@
 // lo*hi + hi*lo + hi*hi (the lo*lo term is below the fp16x2 resolution) accumulate in fp32 on
 // v_mfma_f32_16x16x32_f16 and go to the ring unscaled; the wavefront runs in double and applies
 // the pair's scale m = kscale 4^-order 2^-(ex+ey) (x_scale y_scale) through its coefficients
-// (Arith::Mixed). Relative error against the float64 reference: ~1e-6 on Brownian paths.
+// (Arith::Mixed). Relative error against the float64 reference: ~3e-7 on Brownian paths.
 //
 // Precision "bf16x3": the same scaled increments split into three bf16 parts (hi, mid, lo: each
 // the bf16 rounding of what the previous parts leave), the six products of magnitude >= 2^-16 of

@@ -132,7 +132,7 @@ class SigPDE:
                   per-cell arithmetic for float32 inputs, all-double for float64 inputs;
       "fp16x2" -- linear kernel only: increments <dx, dy> formed from the paths and multiplied as
                   per-path-scaled fp16 hi/lo pairs on MFMA (fp32 accumulate), double wavefront;
-                  relative error ~1e-6 against the float64 reference scheme;
+                  relative error ~3e-7 against the float64 reference scheme;
       "bf16x3" -- linear kernel only: as "fp16x2" with the increments split into three bf16 parts
                   and six bf16 MFMA products (the terms of magnitude >= 2^-16 of hi*hi);
       "fp32"   -- linear kernel only, plain FP32 fused: increments rounded to float, <dx, dy> on
