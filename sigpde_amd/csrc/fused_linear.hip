@@ -6,8 +6,9 @@
 //
 // Precision "fp16x2": each path's increments are formed exactly in double, scaled by a per-path
 // power of two 2^e (so max |dx| * 2^e lies in [2^13, 2^14)) and split into fp16 hi + lo. Products
-// lo*hi + hi*lo + hi*hi (the lo*lo term is below the fp16x2 resolution) accumulate in fp32 on
-// v_mfma_f32_16x16x32_f16 and go to the ring unscaled; the wavefront runs in double and applies
+// lo*hi + hi*lo and hi*hi (the lo*lo term is below the fp16x2 resolution) accumulate in fp32 on
+// v_mfma_f32_16x16x32_f16, in one accumulator per magnitude joined by a float add, and go to the
+// ring unscaled; the wavefront runs in double and applies
 // the pair's scale m = kscale 4^-order 2^-(ex+ey) (x_scale y_scale) through its coefficients
 // (Arith::Mixed). Relative error against the float64 reference: ~3e-7 on Brownian paths.
 //
