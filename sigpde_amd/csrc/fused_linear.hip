@@ -334,12 +334,10 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
     const int N = (int)((Ly - 1) << order);
     auto go = [&](auto rc) -> hipError_t {
       constexpr int R = decltype(rc)::value;
-      bool gl = false;
-      const size_t shmem = fused_line_placement<A>(
+      const bool gl = fused_lines_in_global<A>(
           reinterpret_cast<const void *>(&k_linear_gram_fused<T, KB, PREC, R, false>),
-          reinterpret_cast<const void *>(&k_linear_gram_fused<T, KB, PREC, R, true>), N, grid, st,
-          gl);
-      return with_fused_lines(gl, (size_t)grid * r16_line_bytes<A>(N), st, [&](void *lines) {
+          reinterpret_cast<const void *>(&k_linear_gram_fused<T, KB, PREC, R, true>), N, grid, st);
+      return with_fused_lines<A>(gl, N, grid, st, [&](void *lines, size_t shmem) {
         if (lines)
           k_linear_gram_fused<T, KB, PREC, R, true><<<g3, b3, shmem, st>>>(a, sxd, syd, lines);
         else

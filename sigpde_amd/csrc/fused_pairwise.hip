@@ -62,12 +62,11 @@ hipError_t launch_linear_pairwise_fused(const GramFusedArgs<T> &a, const double 
   const int N = a.n << a.order;
   auto go = [&](auto rc) -> hipError_t {
     constexpr int R = decltype(rc)::value;
-    bool gl = false;
-    const size_t shmem = fused_line_placement<A>(
+    const bool gl = fused_lines_in_global<A>(
         reinterpret_cast<const void *>(&k_linear_pairwise_fused<T, KB, PREC, R, false>),
         reinterpret_cast<const void *>(&k_linear_pairwise_fused<T, KB, PREC, R, true>), N, waves,
-        st, gl);
-    return with_fused_lines(gl, (size_t)waves * r16_line_bytes<A>(N), st, [&](void *lines) {
+        st);
+    return with_fused_lines<A>(gl, N, waves, st, [&](void *lines, size_t shmem) {
       if (lines)
         k_linear_pairwise_fused<T, KB, PREC, R, true><<<grid, block, shmem, st>>>(a, sxd, syd,
                                                                                   lines);

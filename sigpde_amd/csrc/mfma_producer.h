@@ -304,41 +304,45 @@ struct FusedArith {
 // (measured: 12 instead of 9 waves per CU was 12% slower, 12 instead of 7 was 7% faster).
 // Results are the same either way. A stream under graph capture keeps the lines in LDS (they
 // always fit there below the fused column limit), so that a captured launch allocates nothing.
-// k_lds / k_gl: the kernel of each placement. Returns the launch's LDS bytes; sets gl.
+// k_lds / k_gl: the kernel of each placement. Returns true for lines in global memory.
 constexpr size_t kFusedLineBytes = (size_t)4 << 30;
 template <Arith A>
-inline size_t fused_line_placement(const void *k_lds, const void *k_gl, int N, int64_t waves,
-                                   hipStream_t st, bool &gl) {
-  const size_t lds_lines = r16_lds_bytes<A>(N), lds_ring = r16_ring_bytes<A>();
-  gl = false;
+inline bool fused_lines_in_global(const void *k_lds, const void *k_gl, int N, int64_t waves,
+                                  hipStream_t st) {
+  const size_t lds_lines = r16_lds_bytes<A>(N);
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(st, &cap) != hipSuccess) {
     (void)hipGetLastError();  // a failed query only keeps the lines in LDS
-    return lds_lines;
+    return false;
   }
-  if (cap != hipStreamCaptureStatusNone) return lds_lines;
-  if ((double)waves * (double)r16_line_bytes<A>(N) > (double)kFusedLineBytes) return lds_lines;
+  if (cap != hipStreamCaptureStatusNone) return false;
+  if ((double)waves * (double)r16_line_bytes<A>(N) > (double)kFusedLineBytes) return false;
   const int occ_lds = lds_lines <= kMaxLdsBytes ? blocks_per_cu(k_lds, 64, lds_lines) : 0;
-  const int occ_gl = blocks_per_cu(k_gl, 64, lds_ring);
-  gl = waves > (int64_t)cu_count() * occ_lds && occ_gl > occ_lds && 2 * occ_gl >= 3 * occ_lds;
-  return gl ? lds_ring : lds_lines;
+  const int occ_gl = blocks_per_cu(k_gl, 64, r16_ring_bytes<A>());
+  return waves > (int64_t)cu_count() * occ_lds && occ_gl > occ_lds && 2 * occ_gl >= 3 * occ_lds;
 }
 
-// launch(lines): lines null, or (gl) a block of `bytes` allocated stream-ordered on st for the
-// launch and freed after it on the same stream.
-template <class F>
-inline hipError_t with_fused_lines(bool gl, size_t bytes, hipStream_t st, F &&launch) {
-  if (!gl) {
-    launch(nullptr);
-    return hipGetLastError();
+// launch(lines, shmem): lines null with the lines in shmem = r16_lds_bytes bytes of LDS, or (gl)
+// one block of r16_line_bytes per wave, allocated stream-ordered on st for the launch and freed
+// after it on the same stream, with shmem = r16_ring_bytes. When that block cannot be allocated
+// and the lines fit in LDS, the launch keeps them in LDS (same results, fewer waves per CU).
+template <Arith A, class F>
+inline hipError_t with_fused_lines(bool gl, int N, int64_t waves, hipStream_t st, F &&launch) {
+  const size_t lds_lines = r16_lds_bytes<A>(N);
+  if (gl) {
+    void *lines = nullptr;
+    hipError_t e = hipMallocAsync(&lines, (size_t)waves * r16_line_bytes<A>(N), st);
+    if (e == hipSuccess) {
+      launch(lines, r16_ring_bytes<A>());
+      e = hipGetLastError();
+      const hipError_t ef = hipFreeAsync(lines, st);
+      return e != hipSuccess ? e : ef;
+    }
+    if (lds_lines > kMaxLdsBytes) return e;
+    (void)hipGetLastError();  // clears the failed allocation's error before the LDS launch
   }
-  void *lines = nullptr;
-  hipError_t e = hipMallocAsync(&lines, bytes, st);
-  if (e != hipSuccess) return e;
-  launch(lines);
-  e = hipGetLastError();
-  const hipError_t ef = hipFreeAsync(lines, st);
-  return e != hipSuccess ? e : ef;
+  launch(nullptr, lds_lines);
+  return hipGetLastError();
 }
 
 // This wave's block of global boundary lines (GL launches), else null.
