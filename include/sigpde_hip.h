@@ -2,7 +2,8 @@
  * sigpde_hip.h -- C ABI of the MI355X (gfx950) signature-kernel PDE solver, libsigpde_hip.so.
  *
  * Conventions
- *   - Every pointer is a device pointer owned by the caller, except where stated.
+ *   - Every pointer is a device pointer owned by the caller, except where stated. A call whose
+ *     result is empty (a batch of 0) returns 0 at once; its pointers may then be NULL.
  *   - Arrays are C-contiguous unless a stride argument is given. Strides count elements.
  *   - dtype: SIGPDE_F32 (float storage, the reference's float32 semantics) or SIGPDE_F64.
  *   - Work is enqueued on `stream` (a hipStream_t; NULL = the default stream) and is

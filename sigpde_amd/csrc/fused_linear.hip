@@ -430,8 +430,10 @@ static int check_fused(int dtype, int precision, const void *X, const void *Y, i
     return set_error(SIGPDE_EINVAL, "linear: unknown dtype %d", dtype);
   if (!known_precision(precision))
     return set_error(SIGPDE_EINVAL, "linear: unsupported precision mode %d", precision);
-  if (Bx < 0 || By < 0 || !X || !Y || !out || !ws)
-    return set_error(SIGPDE_EINVAL, "linear: bad batch or null pointer");
+  if (Bx < 0 || By < 0) return set_error(SIGPDE_EINVAL, "linear: negative batch");
+  // an empty result reads and writes nothing: its pointers may be null (empty device tensors)
+  if (Bx > 0 && By > 0 && (!X || !Y || !out || !ws))
+    return set_error(SIGPDE_EINVAL, "linear: null pointer");
   if (Lx < 2 || Ly < 2) return set_error(SIGPDE_EINVAL, "linear: paths need length >= 2");
   if (d < 1 || d > kMaxDim) return set_error(SIGPDE_EINVAL, "linear: dim must be in [1, %d]", (int)kMaxDim);
   if (precision == SIGPDE_PREC_FP64 && d > kMaxDimFp64)

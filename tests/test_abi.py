@@ -63,3 +63,25 @@ def test_argument_validation_without_device(lib):
     assert "normalizer" in L.sigpde_last_error().decode()
     # empty work is a no-op success
     assert L.sigpde_pde_pairwise(0, None, 0, 5, 5, 25, 0, None, None, None, None) == 0
+
+
+def test_fused_entry_points_empty_and_negative_batches_without_device(lib):
+    """The fused linear entry points return at once for an empty result, with null pointers (what
+    empty device tensors carry), and reject a negative batch; neither touches the device."""
+    L = _native.load()
+    for prec in (1, 2, 3, 4):
+        # pairwise: dtype, precision, X, Y, B, Lx, Ly, d, kscale, order, sx, sy, out, ws, bytes, stream
+        assert L.sigpde_linear_pairwise(0, prec, None, None, 0, 7, 7, 3, 1.0, 1, None, None, None,
+                                        None, 0, None) == 0
+        # gram: dtype, precision, X, Bx, Lx, Y, By, Ly, d, kscale, order, sx, sy, symmetric, out,
+        #       ldo, ws, bytes, stream
+        assert L.sigpde_linear_gram(1, prec, None, 0, 7, None, 4, 6, 3, 1.0, 1, None, None, 0,
+                                    None, 4, None, 0, None) == 0
+        assert L.sigpde_linear_gram(0, prec, None, 4, 7, None, 0, 6, 3, 1.0, 0, None, None, 0,
+                                    None, 0, None, 0, None) == 0
+        assert L.sigpde_linear_pairwise(0, prec, None, None, -1, 7, 7, 3, 1.0, 1, None, None, None,
+                                        None, 0, None) == 1
+        assert "negative batch" in L.sigpde_last_error().decode()
+        assert L.sigpde_linear_gram(0, prec, None, 2, 7, None, 3, 6, 3, 1.0, 0, None, None, 0,
+                                    None, 3, None, 0, None) == 1
+        assert "null pointer" in L.sigpde_last_error().decode()

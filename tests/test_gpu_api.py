@@ -232,3 +232,42 @@ def test_golden_fixtures_through_the_api():
     assert np.max(np.abs(lam - g["lam"])) < 1e-9
     K = rk.gram(cuda(g["X"]), tol=1e-10).cpu().numpy()
     assert rel_fro(K, g["K"]) < 1e-9
+
+
+@pytest.mark.parametrize("prec", ["ref", "fp32", "fp16x2", "bf16x3", "fp64"])
+def test_empty_batches_every_precision(prec):
+    """A batch of no paths gives an empty result of the right shape on every method, precision
+    and dtype (no launch with an empty grid)."""
+    lin = sig.kernels.LinearKernel()
+    for dt in (torch.float32, torch.float64):
+        e = torch.zeros((0, 7, 3), device=DEV, dtype=dt)
+        y = torch.rand((4, 6, 3), device=DEV, dtype=dt)
+        k = sig.SigPDE(lin, 1, precision=prec)
+        assert k.pairwise(e, e).shape == (0,)
+        assert k.gram(e, y).shape == (0, 4)
+        assert k.gram(y, e).shape == (4, 0)
+        assert k.gram(e).shape == (0, 0)
+        rk = sig.RobustSigPDE(lin, 1, precision=prec)
+        assert rk.normalization(e).shape == (0,)
+        assert rk.pairwise(e, e).shape == (0,)
+        assert rk.gram(e, y).shape == (0, 4)
+        torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("prec,bar", [("ref", 1e-10), ("fp32", 1e-5), ("fp16x2", 1e-6),
+                                      ("bf16x3", 1e-6), ("fp64", 1e-10)])
+def test_single_increment_paths(prec, bar):
+    """Paths of length 2 (one increment each), with lengths unequal to the other side's, at orders
+    0 and 2, against the float64 oracle (float64 inputs; float32 output rounding on the fused
+    mixed and fp32 paths)."""
+    lin = sig.kernels.LinearKernel()
+    x = brownian_np(5, 2, 3, 0.5, 61)
+    y = brownian_np(4, 9, 3, 0.5, 62)
+    for order in (0, 2):
+        k = sig.SigPDE(lin, order, precision=prec)
+        got = k.gram(cuda(x), cuda(y)).double().cpu().numpy()
+        assert rel_fro(got, O.sig_gram(x, y, order)) < bar, (prec, order)
+        got = k.pairwise(cuda(x[:4]), cuda(y)).double().cpu().numpy()
+        assert rel_fro(got, O.sig_pairwise(x[:4], y, order)) < bar, (prec, order)
+        got = k.gram(cuda(y), cuda(x)).double().cpu().numpy()
+        assert rel_fro(got, O.sig_gram(y, x, order)) < bar, (prec, order)
