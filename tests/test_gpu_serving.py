@@ -191,3 +191,25 @@ def test_warmup_serves_every_kind_and_forgets_it():
     reqs = S.synthetic_requests(4, seed=3, device=DEV, max_batch_size=4, lengths=(16, 32))
     srv.run(reqs, realtime=False)
     assert all(r.error is None for r in reqs) and len(srv.completed) == 4
+
+
+def test_empty_requests_complete_with_empty_results():
+    """Requests with a batch of no paths, of every kind, in a mix with ordinary ones: each completes
+    with an empty result of the direct call's shape, and the ordinary ones are unaffected."""
+    reqs = []
+    for i, kind in enumerate(S.KINDS):
+        e = quantized(0, 9, 3, 70 + i)
+        y = None if kind in ("normalization", "gram", "robust_gram") else e
+        prec = "fp16x2" if kind in ("pairwise", "gram") else "ref"
+        reqs.append(S.Request(kind, e, y, dyadic_order=1, precision=prec))
+        x = quantized(3, 9, 3, 80 + i)
+        reqs.append(S.Request(kind, x, None if y is None else quantized(3, 9, 3, 90 + i),
+                              dyadic_order=1, precision=prec))
+    ref = [alone(r) for r in reqs]
+    srv = S.Server(DEV, streams=2)
+    srv.run(reqs, realtime=False)
+    torch.cuda.synchronize()
+    for r, a in zip(reqs, ref):
+        assert r.error is None, (r.kind, r.error)
+        assert r.result.shape == a.shape, r.kind
+        torch.testing.assert_close(r.result, a, rtol=2e-4, atol=1e-6)
