@@ -13,7 +13,10 @@ Timed beside it on the same inputs:
 Besides the JSON contract fields the line carries:
   rel_err_fp64   ||K - K64||_F / ||K64||_F on a sampled block, K64 = the float64 oracle on the same
                  float32 inputs upcast (computed in the CPU-baseline leg, which runs the oracle);
-                 the same figure is given for the two comparison paths
+                 the same figure is given for the two comparison paths; per entry:
+                 rel_err_fp64_max / _p99 (|K - K64| / |K64|), the |K64| of the worst entry and
+                 the largest |K - K64| against the block's RMS entry (entries that cancel to near
+                 zero carry large relative but small absolute errors)
   rel_err_fp64_full  the same norm over all entries, K64 = precision="ref" on the float64 inputs
                  (the reference scheme in float64 on the GPU)
   latency_ms     p50 / p99 of the per-step times
@@ -257,6 +260,10 @@ def cpu_leg(args, X, Y, K, others):
         "rel_err_fp64": rel,
         "rel_err_fp64_max": float(per.max()),
         "rel_err_fp64_p99": float(np.percentile(per, 99)),
+        # the entry of largest relative error: |K64| there against the block's RMS entry
+        "rel_err_fp64_max_at_abs_k": float(np.abs(K64).flat[int(np.argmax(per))]),
+        "k64_rms": float(np.sqrt(np.mean(K64 * K64))),
+        "abs_err_fp64_max_over_rms": float(np.abs(Kg - K64).max() / np.sqrt(np.mean(K64 * K64))),
         "rel_err_sample": f"{s}x{s} pairs",
     }, rel_others, {
         "value": side2 * side2 / dt2,
