@@ -235,7 +235,7 @@ def cpu_leg(args, X, Y, K, others):
         Kr = Ko[:s, :s].double().cpu().numpy()
         rel_others[name] = float(np.linalg.norm(Kr - K64) / np.linalg.norm(K64))
 
-    cores = min(os.cpu_count() or 1, 16)
+    cores, host = _cpu_share()
     oracle.set_num_threads(cores)
     X32 = X.cpu().numpy()
     Y32 = Y.cpu().numpy()
@@ -274,7 +274,34 @@ def cpu_leg(args, X, Y, K, others):
                   f"(C restatement, OpenMP over pairs), {dt2:.1f} s",
         "value_1_thread": side1 * side1 / dt1,
         "sample_1_thread": f"{side1}x{side1} pairs on one thread, {dt1:.1f} s",
+        **host,
     }
+
+
+def _cpu_share():
+    """Threads for the CPU baseline and a description of the host: the GPU box gives one GPU's
+    job a share of 16 host threads (OMP_NUM_THREADS is set to it there), whatever nproc reports,
+    so the baseline uses min(affinity, OMP_NUM_THREADS or 16) threads and records nproc, the
+    affinity count and the CPU model beside it."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = nproc
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    cap = int(omp) if omp.isdigit() and int(omp) > 0 else 16
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cores = max(1, min(affinity, cap))
+    return cores, {"nproc": nproc, "affinity_cpus": affinity, "cpu_model": model,
+                   "threads_cap": f"{cap} (OMP_NUM_THREADS)" if omp else f"{cap} (default)"}
 
 
 def _rel(K, K64):
@@ -319,7 +346,7 @@ def workloads_leg(args, dev):
     """
     import oracle
 
-    oracle.set_num_threads(min(os.cpu_count() or 1, 16))
+    oracle.set_num_threads(_cpu_share()[0])
     lin = sig.kernels.LinearKernel()
     out = {}
 

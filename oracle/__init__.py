@@ -6,7 +6,8 @@ imports it and has no CPU fallback.
 
 Contents:
   * ctypes bindings to ``liboracle.so`` (``sigpde_oracle.c``): the PDE solve in FP64 as written and
-    in the reference's FP32 semantics, Gram / symmetric / scaled variants, Chandrupatla;
+    in the reference's FP32 semantics, Gram / symmetric / scaled variants, Chandrupatla; pde_step
+    as written (default) or with NVVM's FMA contraction (``contracted()``);
   * NumPy restatements of the static kernels (sigpde/torch/static_kernels.py:3-54), the increment
     second difference (sigpde/torch/kernel.py:13-21), the default normalizer
     (sigpde/torch/kernel.py:23-24) and the robust normalisation pipeline
@@ -16,7 +17,9 @@ Contents:
     used to cross-check the C restatement bit for bit.
 
 Parity pins for every function here live in ``tests/test_oracle.py`` (closed forms, invariants,
-brute-force signatures, scipy root-finding, golden fixtures).
+brute-force signatures, scipy root-finding, golden fixtures) and ``tests/test_oracle_typing.py``
+(known answers of the float32 typing rules and of both contraction settings, computed in exact
+rational arithmetic with explicit rounding).
 """
 from __future__ import annotations
 
@@ -52,6 +55,7 @@ def lib():
         L = ctypes.CDLL(_LIB_PATH)
         for name in ("orc_pairwise_f64", "orc_pairwise_f32"):
             getattr(L, name).argtypes = [_ptr, _i64, _i64, _i64, _int, _ptr, _ptr, _ptr]
+        L.orc_pairwise_f32_dscale.argtypes = [_ptr, _i64, _i64, _i64, _int, _ptr, _ptr]
         for name in ("orc_gram_f64", "orc_gram_f32"):
             getattr(L, name).argtypes = [_ptr, _i64, _i64, _i64, _i64, _int, _ptr, _ptr, _int, _ptr, _i64]
         for name in ("orc_linear_inc_f64", "orc_linear_inc_f32"):
@@ -63,8 +67,38 @@ def lib():
             getattr(L, name).argtypes = [_ptr, _i64, _i64, _int, _ptr, _ptr, _int, _dbl, _ptr, _ptr]
         L.orc_num_threads.restype = _int
         L.orc_set_num_threads.argtypes = [_int]
+        L.orc_set_contract.argtypes = [_int]
+        L.orc_get_contract.restype = _int
         _lib = L
     return _lib
+
+
+def set_contract(on: bool) -> None:
+    """Contraction setting of pde_step for every later solve: False (default) rounds every
+    product and sum as written; True applies NVVM's FMA contraction of the expression
+    (A = fma(c2, 1/12, fma(c, 0.5, 1)), B = fma(c2, -1/12, 1), k = fma(k01 + k10, A, -(k00 B))),
+    the arithmetic of the reference on its NVIDIA target."""
+    lib().orc_set_contract(int(bool(on)))
+
+
+def get_contract() -> bool:
+    return bool(lib().orc_get_contract())
+
+
+class contracted:
+    """Context manager: solves inside it use the contracted pde_step (set_contract(True))."""
+
+    def __init__(self, on: bool = True):
+        self.on = on
+
+    def __enter__(self):
+        self.prev = get_contract()
+        set_contract(self.on)
+        return self
+
+    def __exit__(self, *exc):
+        set_contract(self.prev)
+        return False
 
 
 def set_num_threads(n: int) -> None:
@@ -101,6 +135,18 @@ def pde_pairwise(inc: np.ndarray, order: int, sx=None, sy=None) -> np.ndarray:
     sx, sy = _c(sx, dt), _c(sy, dt)
     out = np.zeros(B, dtype=dt)
     getattr(lib(), "orc_pairwise_" + _sfx(dt))(_p(inc), B, m, n, order, _p(sx), _p(sy), _p(out))
+    return out
+
+
+def pde_pairwise_f32_dscale(inc: np.ndarray, order: int, s: np.ndarray) -> np.ndarray:
+    """The root-find's evaluation F for float32 paths: out[b] = K_b of c = inc[b] * s[b] with the
+    double scale s = xt**2 (c and c**2 in double, cells stored as float32);
+    sigpde/cuda_pairwise_kernels.py:114."""
+    inc = _c(inc, np.float32)
+    B, m, n = inc.shape
+    s = _c(s, np.float64)
+    out = np.zeros(B, dtype=np.float32)
+    lib().orc_pairwise_f32_dscale(_p(inc), B, m, n, order, _p(s), _p(out))
     return out
 
 

@@ -11,6 +11,15 @@
  *     except inc*inc, which the reference's Numba typing keeps in float32; the increment tensor
  *     and the scale product are float32; every stored cell is rounded to float.
  *
+ * Two contraction settings of pde_step (orc_set_contract), for both modes:
+ *   0 (default): as written, every product and sum rounded separately;
+ *   1: the FMA contraction NVVM applies to the expression on an NVIDIA GPU (the reference's
+ *      target; Numba leaves NVVM's contraction on, sigpde never sets fastmath):
+ *        A = fma(c2, 1/12, fma(c, 0.5, 1)),  B = fma(c2, -1/12, 1),
+ *        k = fma(k01 + k10, A, -(k00 * B)).
+ *      fma(c, 0.5, 1) equals 1 + 0.5 c (0.5 c is exact). The root-find's bracket arithmetic
+ *      stays as written in both settings.
+ *
  * Reference files followed (stojl/SigPDE):
  *   sigpde/cuda_device_functions.py:3-5    pde_step
  *   sigpde/cuda_pairwise_kernels.py:8-85   wavefront, boundary k=1, output k[M][N], scaled variant
@@ -33,14 +42,31 @@
 #include <omp.h>
 #endif
 
-/* pde_step, sigpde/cuda_device_functions.py:3-5, FP64 operand order as written:
+/* Contraction setting of pde_step (see the header): set once before a call, read by every
+ * thread. */
+static int g_contract = 0;
+
+void orc_set_contract(int on) { g_contract = on ? 1 : 0; }
+int orc_get_contract(void) { return g_contract; }
+
+/* pde_step, sigpde/cuda_device_functions.py:3-5, on double c and c2 = c**2:
  *   (k01 + k10) * (1. + 0.5 * inc + (1./12) * inc**2) - k00 * (1. - (1./12) * inc**2) */
-static inline double step_f64(double k00, double k01, double k10, double c)
+static inline double step_core(double k00, double k01, double k10, double c, double c2)
 {
-    double c2 = c * c;
+    if (g_contract) {
+        double a = fma(c2, 1.0 / 12, fma(c, 0.5, 1.0));
+        double b = fma(c2, -1.0 / 12, 1.0);
+        return fma(k01 + k10, a, -(k00 * b));
+    }
     double a = (1.0 + 0.5 * c) + (1.0 / 12) * c2;
     double b = 1.0 - (1.0 / 12) * c2;
     return (k01 + k10) * a - k00 * b;
+}
+
+/* FP64: every operand double. */
+static inline double step_f64(double k00, double k01, double k10, double c)
+{
+    return step_core(k00, k01, k10, c, c * c);
 }
 
 /* pde_step under the reference's FP32 typing: inc and inc**2 are float32, the literals and the
@@ -48,9 +74,7 @@ static inline double step_f64(double k00, double k01, double k10, double c)
 static inline double step_f32ref(double k00, double k01, double k10, float c)
 {
     float c2 = c * c;
-    double a = (1.0 + 0.5 * (double)c) + (1.0 / 12) * (double)c2;
-    double b = 1.0 - (1.0 / 12) * (double)c2;
-    return (k01 + k10) * a - k00 * b;
+    return step_core(k00, k01, k10, (double)c, (double)c2);
 }
 
 /* One pair, FP64. inc is m x n with row stride ld; c = inc * scale (sigpde/cuda_pairwise_kernels.py:69).
@@ -141,6 +165,21 @@ void orc_pairwise_f32(const float *inc, int64_t B, int64_t m, int64_t n, int ord
             float s = (sx || sy) ? (sy ? sy[b] : 1.0f) * (sx ? sx[b] : 1.0f) : 1.0f;
             out[b] = pair_f32ref(inc + b * m * n, n, m, n, order, s, buf);
         }
+        free(buf);
+    }
+}
+
+/* The root-find's function evaluation for float32 paths (sigpde/cuda_pairwise_kernels.py:114):
+ * float storage, c = inc * s[b] in double (s = xt**2, a double), c**2 in double. */
+void orc_pairwise_f32_dscale(const float *inc, int64_t B, int64_t m, int64_t n, int order,
+                             const double *s, float *out)
+{
+#pragma omp parallel
+    {
+        float *buf = (float *)malloc(sizeof(float) * 2 * ((n << order) + 1));
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t b = 0; b < B; ++b)
+            out[b] = pair_f32store_f64c(inc + b * m * n, n, m, n, order, s[b], buf);
         free(buf);
     }
 }

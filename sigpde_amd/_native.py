@@ -112,10 +112,28 @@ def _require_gpu(*ts):
             raise ValueError("sigpde_amd kernels take GPU tensors (device='cuda'); got a CPU tensor")
 
 
+def _require(ok: bool, msg: str) -> None:
+    """Layout checks that stay on under `python -O`: a wrong stride or size would reach the
+    kernels as out-of-bounds device accesses."""
+    if not ok:
+        raise ValueError(msg)
+
+
+def _require_vec(t, n: int, what: str) -> None:
+    """An optional per-path vector: None, or a contiguous 1-D tensor of n elements."""
+    if t is not None:
+        _require(t.dim() == 1 and t.shape[0] == n and (n <= 1 or t.stride(0) == 1),
+                 f"{what} must be a contiguous vector of {n} elements, got {tuple(t.shape)}")
+
+
 def pde_pairwise(inc: torch.Tensor, order: int, out: torch.Tensor, sx=None, sy=None) -> None:
     """out[b] = PDE solve of increments inc[b] (B, m, n)."""
     _require_gpu(inc, out, sx, sy)
-    assert inc.dim() == 3 and inc.stride(2) == 1 and inc.stride(1) == inc.shape[2]
+    _require(inc.dim() == 3 and inc.stride(2) == 1 and inc.stride(1) == inc.shape[2],
+             "pde_pairwise: increments must be (B, m, n) with unit-stride rows packed m x n")
+    _require_vec(out, inc.shape[0], "pde_pairwise: out")
+    _require_vec(sx, inc.shape[0], "pde_pairwise: sx")
+    _require_vec(sy, inc.shape[0], "pde_pairwise: sy")
     B, m, n = inc.shape
     rc = load().sigpde_pde_pairwise(dtype_code(inc.dtype), _ptr(inc), B, m, n, inc.stride(0),
                                     int(order), _ptr(sx), _ptr(sy), _ptr(out), _stream(inc.device))
@@ -126,8 +144,16 @@ def pde_gram(inc: torch.Tensor, order: int, out: torch.Tensor, off_x: int = 0, o
              sx=None, sy=None, symmetric: bool = False) -> None:
     """Solves the (bx, by) pairs of an increment tile into out[off_x + i, off_y + j]."""
     _require_gpu(inc, out, sx, sy)
-    assert inc.dim() == 4 and inc.stride(3) == 1 and inc.stride(2) == inc.shape[3]
-    assert out.dim() == 2 and out.stride(1) == 1
+    _require(inc.dim() == 4 and inc.stride(3) == 1 and inc.stride(2) == inc.shape[3],
+             "pde_gram: increments must be (bx, by, m, n) with unit-stride rows packed m x n")
+    _require(out.dim() == 2 and out.stride(1) == 1, "pde_gram: out must be 2-D with unit column stride")
+    _require(off_x >= 0 and off_y >= 0 and off_x + inc.shape[0] <= out.shape[0]
+             and off_y + inc.shape[1] <= out.shape[1], "pde_gram: tile does not fit in out")
+    if symmetric:
+        _require(off_y + inc.shape[1] <= out.shape[0] and off_x + inc.shape[0] <= out.shape[1],
+                 "pde_gram: the mirrored tile does not fit in out")
+    _require_vec(sx, inc.shape[0], "pde_gram: sx")
+    _require_vec(sy, inc.shape[1], "pde_gram: sy")
     bx, by, m, n = inc.shape
     rc = load().sigpde_pde_gram(dtype_code(inc.dtype), _ptr(inc), bx, by, m, n, inc.stride(0),
                                 inc.stride(1), int(order), _ptr(sx), _ptr(sy), int(symmetric),
@@ -141,7 +167,11 @@ def robust_norm(inc: torch.Tensor, order: int, lam: torch.Tensor, knorm: torch.T
     """Chandrupatla normalisation per path. target=None fuses the log normaliser and writes
     K(x,x) into knorm; otherwise knorm and target are inputs."""
     _require_gpu(inc, lam, knorm, target, iters)
-    assert inc.dim() == 3 and inc.shape[1] == inc.shape[2] and inc.stride(2) == 1
+    _require(inc.dim() == 3 and inc.shape[1] == inc.shape[2] and inc.stride(2) == 1
+             and inc.stride(1) == inc.shape[2],
+             "robust_norm: increments must be (B, m, m) with unit-stride rows packed m x m")
+    for t, name in ((lam, "lambda"), (knorm, "knorm"), (target, "target"), (iters, "iters")):
+        _require_vec(t, inc.shape[0], f"robust_norm: {name}")
     B, m, _ = inc.shape
     mode = NORM_LOG if target is None else NORM_GIVEN
     rc = load().sigpde_robust_norm(dtype_code(inc.dtype), _ptr(inc), B, m, inc.stride(0),
@@ -165,7 +195,12 @@ def linear_gram(X: torch.Tensor, Y: torch.Tensor, order: int, kscale: float, out
                 precision: str = "fp16x2", sx=None, sy=None, symmetric: bool = False) -> None:
     """Fused linear-kernel Gram: out[i, j] = K(X_i, Y_j), increments on MFMA."""
     _require_gpu(X, Y, out, sx, sy)
-    assert X.is_contiguous() and Y.is_contiguous() and out.stride(1) == 1
+    _require(X.dim() == 3 and Y.dim() == 3 and X.is_contiguous() and Y.is_contiguous(),
+             "linear_gram: X and Y must be contiguous (batch, length, dim) tensors")
+    _require(out.dim() == 2 and out.stride(1) == 1 and tuple(out.shape) == (X.shape[0], Y.shape[0]),
+             "linear_gram: out must be (Bx, By) with unit column stride")
+    _require_vec(sx, X.shape[0], "linear_gram: sx")
+    _require_vec(sy, Y.shape[0], "linear_gram: sy")
     Bx, Lx, d = X.shape
     By, Ly, _ = Y.shape
     same = X.data_ptr() == Y.data_ptr() and X.shape == Y.shape
@@ -183,7 +218,12 @@ def linear_pairwise(X: torch.Tensor, Y: torch.Tensor, order: int, kscale: float,
                     out: torch.Tensor, precision: str = "fp16x2", sx=None, sy=None) -> None:
     """Fused linear-kernel pairwise: out[b] = K(X_b, Y_b)."""
     _require_gpu(X, Y, out, sx, sy)
-    assert X.is_contiguous() and Y.is_contiguous()
+    _require(X.dim() == 3 and Y.dim() == 3 and X.is_contiguous() and Y.is_contiguous()
+             and X.shape[0] == Y.shape[0],
+             "linear_pairwise: X and Y must be contiguous (batch, length, dim) with one batch")
+    _require_vec(out, X.shape[0], "linear_pairwise: out")
+    _require_vec(sx, X.shape[0], "linear_pairwise: sx")
+    _require_vec(sy, X.shape[0], "linear_pairwise: sy")
     B, Lx, d = X.shape
     Ly = Y.shape[1]
     same = X.data_ptr() == Y.data_ptr() and X.shape == Y.shape
@@ -203,7 +243,8 @@ def rbf_increments(ip: torch.Tensor, ip_strides, A: int, B: int, m: int, n: int,
     """RBF increments (A, B, m, n) from the point inner products ip (element (a, b, i, j) at the
     given element strides) and the row sums of squares xx (a, i), yy (a, b, j)."""
     _require_gpu(ip, xx, yy, out)
-    assert out.is_contiguous() and out.numel() == A * B * m * n
+    _require(out.is_contiguous() and out.numel() == A * B * m * n,
+             "rbf_increments: out must be contiguous with A * B * m * n elements")
     strides = (_i64 * 4)(*[int(v) for v in ip_strides])
     rc = load().sigpde_rbf_increments(dtype_code(ip.dtype), _ptr(ip), A, B, m, n, strides,
                                       _ptr(xx), int(xx_stride_a), _ptr(yy), int(yy_stride_a),
@@ -216,7 +257,11 @@ def l1rbf_increments(X: torch.Tensor, Y: torch.Tensor, sigma: float, order: int,
                      out: torch.Tensor, pairwise: bool) -> None:
     """L1-RBF increments from the paths: (A, m, n) pairwise (X_a with Y_a) or (A, B, m, n)."""
     _require_gpu(X, Y, out)
-    assert X.is_contiguous() and Y.is_contiguous() and out.is_contiguous()
+    _require(X.dim() == 3 and Y.dim() == 3 and X.is_contiguous() and Y.is_contiguous()
+             and out.is_contiguous(), "l1rbf_increments: contiguous (batch, length, dim) inputs")
+    _require(out.numel() == (X.shape[0] * (X.shape[1] - 1) * (Y.shape[1] - 1) if pairwise else
+                             X.shape[0] * Y.shape[0] * (X.shape[1] - 1) * (Y.shape[1] - 1)),
+             "l1rbf_increments: out has the wrong number of elements")
     A, Lx, d = X.shape
     B, Ly, _ = Y.shape
     rc = load().sigpde_l1rbf_increments(dtype_code(X.dtype), _ptr(X), A, _ptr(Y), B, Lx - 1,
@@ -230,7 +275,8 @@ def linear_increments(ip: torch.Tensor, ip_strides, A: int, B: int, m: int, n: i
     """Linear-kernel increments (A, B, m, n) of kscale * ip (element (a, b, i, j) at the given
     element strides): the reference scheme's second difference in one pass."""
     _require_gpu(ip, out)
-    assert out.is_contiguous() and out.numel() == A * B * m * n
+    _require(out.is_contiguous() and out.numel() == A * B * m * n,
+             "linear_increments: out must be contiguous with A * B * m * n elements")
     strides = (_i64 * 4)(*[int(v) for v in ip_strides])
     rc = load().sigpde_linear_increments(dtype_code(ip.dtype), _ptr(ip), A, B, m, n, strides,
                                          float(kscale), int(order), _ptr(out), _stream(ip.device))

@@ -110,9 +110,9 @@ template <>
 struct ArithTraits<Arith::Mixed> { using RT = float; static constexpr bool kRound = false; };
 
 // A = (1 + c/2) + c^2/12 and B = 1 - c^2/12 as pde_step writes them, with the products contracted
-// into FMAs (as NVVM contracts the reference's expression). The contractions are written out so
-// that every kernel instantiation rounds a cell the same way: a pair's result does not depend on
-// which launch shape solved it.
+// into FMAs as NVVM contracts the reference's expression (fma(c, 0.5, 1) equals 1 + 0.5 c: the
+// product is exact). The contractions are written out so that every kernel instantiation rounds
+// a cell the same way: a pair's result does not depend on which launch shape solved it.
 template <Arith A>
 __device__ __forceinline__ void coefs(typename ArithTraits<A>::RT c, double s, double h, double q,
                                       double &cA, double &cB) {
@@ -139,12 +139,21 @@ __device__ __forceinline__ void coefs(typename ArithTraits<A>::RT c, double s, d
   cB = fma(c2, -1.0 / 12, 1.0);
 }
 
-// k = (k01 + k10) A - k00 B, grouped as k01*A + (k10*A - k00*B) so that only the last FMA waits
-// for the value shifted in from the row above.
+// k = (k01 + k10) A - k00 B.
+// The reference-scheme modes (Ref*, Scaled*) contract it as NVVM contracts pde_step on the
+// reference's target, fma(k01 + k10, A, -(k00 B)), so that a cell rounds exactly as in the
+// oracle's contracted setting. Mixed (whose increments come from MFMA products and are held only
+// to a tolerance) groups it as k01*A + (k10*A - k00*B), so that only the last FMA waits for the
+// value shifted in from the row above.
 template <Arith A>
 __device__ __forceinline__ double cell(double k01, double k10, double k00, double cA, double cB) {
-  const double w = fma(k10, cA, -(k00 * cB));
-  double r = fma(k01, cA, w);
+  double r;
+  if constexpr (A == Arith::Mixed) {
+    const double w = fma(k10, cA, -(k00 * cB));
+    r = fma(k01, cA, w);
+  } else {
+    r = fma(k01 + k10, cA, -(k00 * cB));
+  }
   if constexpr (ArithTraits<A>::kRound) r = (double)(float)r;
   return r;
 }

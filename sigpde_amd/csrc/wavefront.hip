@@ -205,7 +205,9 @@ __device__ __forceinline__ double log_normalizer(double x) {
 // F(x) = PDE(x^2 inc) - target on [0, 1], F(0) = 1 - target, F(1) = K(x,x) - target, the
 // bookkeeping and the interpolation test operation for operation, in double; function values
 // rounded through the path dtype T as the reference's scratch stores them. Stops when the smaller
-// bracket end's |F| < tol (function value only) or after maxit evaluations.
+// bracket end's |F| < tol (function value only) or after maxit evaluations. The bracket
+// arithmetic is not contracted into FMAs (every product and quotient rounded as written), so that
+// with the same function values the iterates, and lambda, equal the oracle's bit for bit.
 template <typename T>
 struct RootFind {
   double av, cv, bv, fa, fb, fc, t, outv;
@@ -223,9 +225,13 @@ struct RootFind {
     outv = 0.0;
     it = 0;
   }
-  __device__ double next() const { return av + t * (bv - av); }
+  __device__ double next() const {
+#pragma clang fp contract(off)
+    return av + t * (bv - av);
+  }
   // Takes F's evaluation at xt = next() (the wavefront result sol); true when the search stops.
   __device__ bool update(double xt, double sol, double tol, int maxit) {
+#pragma clang fp contract(off)
     ++it;
     T solT = (T)sol;
     double ft = isnan(solT) ? (double)INFINITY : (double)(T)(solT - Tt);
@@ -387,7 +393,9 @@ constexpr size_t kGlobalLineBytes = (size_t)4 << 30;
 
 // Launches a 16-lane kernel: launch(integral_constant R, bool_constant GL, w0, waves, lines) with
 // R from the dyadic order. global: lines in global memory, allocated stream-ordered for the call,
-// in launches of as many waves as kGlobalLineBytes holds (at least 1024).
+// in launches of as many waves as kGlobalLineBytes holds (at least one: the block never exceeds
+// that cap by more than one wave's lines: r16_line_bytes is at most 134 MB, double lines at 2^22
+// refined columns).
 template <Arith A, class F>
 static hipError_t launch16(int order, int N, int64_t waves, bool global, hipStream_t st,
                            F &&launch) {
@@ -405,7 +413,7 @@ static hipError_t launch16(int order, int N, int64_t waves, bool global, hipStre
   }
   const size_t per = r16_line_bytes<A>(N);
   int64_t chunk = (int64_t)(kGlobalLineBytes / per);
-  if (chunk < 1024) chunk = 1024;
+  if (chunk < 1) chunk = 1;
   if (chunk > waves) chunk = waves;
   void *gl = nullptr;
   hipError_t e = hipMallocAsync(&gl, (size_t)chunk * per, st);

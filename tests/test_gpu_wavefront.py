@@ -129,6 +129,16 @@ def test_bad_arguments_raise():
         _native.pde_pairwise(big, 0, out[:1])  # 2^22 + 1 refined columns
     with pytest.raises(TypeError):
         _native.pde_pairwise(inc.half(), 0, out)
+    # layout checks are ValueErrors (they stay on under python -O)
+    with pytest.raises(ValueError, match="unit-stride"):
+        _native.pde_pairwise(inc.transpose(1, 2), 0, out)
+    with pytest.raises(ValueError, match="out must be a contiguous vector"):
+        _native.pde_pairwise(inc, 0, out[:1])
+    with pytest.raises(ValueError, match="does not fit"):
+        _native.pde_gram(inc.reshape(1, 2, 5, 5), 0, torch.zeros((1, 1), device=DEV))
+    X = torch.zeros((2, 6, 3), dtype=torch.float32, device=DEV)
+    with pytest.raises(ValueError, match="contiguous"):
+        _native.linear_gram(X.transpose(0, 1), X, 0, 1.0, torch.zeros((6, 2), device=DEV))
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
