@@ -144,6 +144,24 @@ int sigpde_linear_pairwise(int dtype, int precision, const void *X, const void *
                            size_t workspace_bytes, void *stream);
 
 /*
+ * Up to 64 Grams of one shape class (dtype, precision, lengths Lx / Ly, d, order, kscale) in one
+ * launch: several calls of SigPDE.gram (sigpde/torch/kernel.py:66-124 for the linear kernel),
+ * coalesced by a serving loop. X holds nx paths and Y ny paths, C-contiguous; requests is a HOST
+ * array of nreq rows (x0, bx, y0, by, symmetric): request r is the Gram of X paths x0 .. x0+bx-1
+ * against Y paths y0 .. y0+by-1 (symmetric != 0: those Y paths equal the X paths, bx == by; the
+ * lower triangle is solved and mirrored). Results go to out, request after request, each a
+ * row-major (bx, by) block (request r at the sum of the earlier bx * by). sx (nx,), sy (ny,):
+ * optional scales. Workspace: sigpde_linear_workspace_bytes(precision, nx, Lx, ny, Ly, d, 0).
+ * Each pair is solved exactly as sigpde_linear_gram solves it: results equal that call's bit
+ * for bit.
+ */
+int sigpde_linear_gram_batch(int dtype, int precision, const void *X, int64_t nx, int64_t Lx,
+                             const void *Y, int64_t ny, int64_t Ly, int64_t d, double kscale,
+                             int order, const int64_t *requests, int64_t nreq, const void *sx,
+                             const void *sy, void *out, void *workspace, size_t workspace_bytes,
+                             void *stream);
+
+/*
  * Increments of the non-linear static kernels in one pass. These replace the chain
  * RBFKernel / L1RBFKernel .pairwise / .gram (sigpde/torch/static_kernels.py:13-54) followed by
  * pairwise_inner_product / gram_inner_product (sigpde/torch/kernel.py:13-21):

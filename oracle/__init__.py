@@ -64,7 +64,8 @@ def lib():
             getattr(L, name).argtypes = [_ptr, _ptr, _i64, _i64, _i64, _i64, _i64, _dbl, _int,
                                          _ptr, _ptr, _ptr, _i64]
         for name in ("orc_chandrupatla_f64", "orc_chandrupatla_f32"):
-            getattr(L, name).argtypes = [_ptr, _i64, _i64, _int, _ptr, _ptr, _int, _dbl, _ptr, _ptr]
+            getattr(L, name).argtypes = [_ptr, _i64, _i64, _int, _ptr, _ptr, _int, _dbl, _ptr, _ptr,
+                                         _ptr]
         L.orc_num_threads.restype = _int
         L.orc_set_num_threads.argtypes = [_int]
         L.orc_set_contract.argtypes = [_int]
@@ -188,18 +189,24 @@ def linear_gram(X: np.ndarray, Y: np.ndarray, order: int, kscale: float = 1.0, s
 
 
 def chandrupatla(inc: np.ndarray, order: int, target: np.ndarray, knorm: np.ndarray,
-                 maxit: int = 100, tol: float = 1e-8):
-    """Per-path lambda and evaluation count; sigpde/cuda_pairwise_kernels.py:88-166."""
+                 maxit: int = 100, tol: float = 1e-8, fixed_points: bool = False):
+    """Per-path lambda and evaluation count; sigpde/cuda_pairwise_kernels.py:88-166.
+    fixed_points=True also returns, per path, the first evaluation count after which an iteration
+    left the whole bracket state unchanged (-1 if none): the iteration repeats from there, so
+    lambda is already final."""
     dt = inc.dtype
     inc = _c(inc, dt)
     B, m, m2 = inc.shape
-    assert m == m2
+    if m != m2:
+        raise ValueError("chandrupatla takes (B, m, m) self-increments")
     target, knorm = _c(target, dt), _c(knorm, dt)
     out = np.zeros(B, dtype=dt)
     iters = np.zeros(B, dtype=np.int32)
+    fixed = np.full(B, -1, dtype=np.int32)
     getattr(lib(), "orc_chandrupatla_" + _sfx(dt))(_p(inc), B, m, order, _p(target), _p(knorm),
-                                                   int(maxit), float(tol), _p(out), _p(iters))
-    return out, iters
+                                                   int(maxit), float(tol), _p(out), _p(iters),
+                                                   _p(fixed) if fixed_points else None)
+    return (out, iters, fixed) if fixed_points else (out, iters)
 
 
 # ----------------------------------------------------------------------------- static kernels
@@ -264,18 +271,19 @@ def sig_gram(X, Y, order, kind="linear", param=1.0, sx=None, sy=None, symmetric=
 
 
 def norm_factors(inc: np.ndarray, order: int, normalizer: Optional[Callable] = None,
-                 tol: float = 1e-8, maxit: int = 100):
+                 tol: float = 1e-8, maxit: int = 100, fixed_points: bool = False):
     """sigpde/torch/kernel.py:180-195 for increments (B, m, m) of each path with itself:
     K = PDE(inc); K[isnan] = inf; T = normalizer(K); lambda = Chandrupatla; lambda[K < 1] = 0.
-    Returns (lambda, K, iterations)."""
+    Returns (lambda, K, iterations), and the fixed points of chandrupatla() as a fourth value
+    with fixed_points=True."""
     normalizer = log_normalizer if normalizer is None else normalizer
     knorm = pde_pairwise(inc, order)
     knorm[np.isnan(knorm)] = np.inf
     with np.errstate(all="ignore"):
         target = np.asarray(normalizer(knorm), dtype=inc.dtype)
-    lam, iters = chandrupatla(inc, order, target, knorm, maxit, tol)
+    lam, iters, fixed = chandrupatla(inc, order, target, knorm, maxit, tol, fixed_points=True)
     lam[knorm < 1] = 0
-    return lam, knorm, iters
+    return (lam, knorm, iters, fixed) if fixed_points else (lam, knorm, iters)
 
 
 def robust_normalization(X, order, kind="linear", param=1.0, normalizer=None, tol=1e-8, maxit=100):

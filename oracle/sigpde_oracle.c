@@ -327,9 +327,13 @@ void orc_linear_gram_f32(const float *X, const float *Y, int64_t Bx, int64_t By,
  * F(xt) is evaluated with c = Delta * xt**2 in double.
  * f32 storage mode (f32 != 0): inc/target/knorm/out are float; fb and ft are float differences
  * promoted to double, as the reference's typing gives for float32 arrays (:97, :132).
- * iters[b] (nullable) receives the number of wavefront evaluations performed. */
+ * iters[b] (nullable) receives the number of wavefront evaluations performed.
+ * fixed[b] (nullable) receives the first evaluation count after which the iteration left its
+ * state (a, b, c, fa, fb, fc, t) unchanged, or -1: from there on every iteration repeats (the
+ * reference keeps iterating to maxit; the output does not change). Reporting it changes nothing
+ * else. */
 static double chandrupatla_one(const void *inc, int64_t m, int order, double T, double K1,
-                               int f32, int maxit, double tol, void *buf, int *iters)
+                               int f32, int maxit, double tol, void *buf, int *iters, int *fixed)
 {
     double a = 0.0, c = 0.0, b = 1.0;
     double fa = 1.0 - T;
@@ -337,7 +341,9 @@ static double chandrupatla_one(const void *inc, int64_t m, int order, double T, 
     double fc = fa, t = 0.5;
     double outv = 0.0; /* the reference's output buffer starts at zero (kernel.py:212,249) */
     int it = 0;
+    if (fixed) *fixed = -1;
     for (; it < maxit;) {
+        const double a0 = a, b0 = b, c0 = c, fa0 = fa, fb0 = fb, fc0 = fc, t0 = t;
         double xt = a + t * (b - a);
         double s = xt * xt;
         double sol;
@@ -376,13 +382,17 @@ static double chandrupatla_one(const void *inc, int64_t m, int order, double T, 
         else
             t = 0.5;
         if (!(0 < t && t < 1)) t = 0.5;
+        if (fixed && *fixed < 0 && a == a0 && b == b0 && c == c0 && fa == fa0 && fb == fb0 &&
+            fc == fc0 && t == t0)
+            *fixed = it;
     }
     if (iters) *iters = it;
     return outv;
 }
 
 void orc_chandrupatla_f64(const double *inc, int64_t B, int64_t m, int order, const double *target,
-                          const double *knorm, int maxit, double tol, double *out, int *iters)
+                          const double *knorm, int maxit, double tol, double *out, int *iters,
+                          int *fixed)
 {
 #pragma omp parallel
     {
@@ -390,13 +400,14 @@ void orc_chandrupatla_f64(const double *inc, int64_t B, int64_t m, int order, co
 #pragma omp for schedule(dynamic, 1)
         for (int64_t b = 0; b < B; ++b)
             out[b] = chandrupatla_one(inc + b * m * m, m, order, target[b], knorm[b], 0, maxit, tol,
-                                      buf, iters ? iters + b : NULL);
+                                      buf, iters ? iters + b : NULL, fixed ? fixed + b : NULL);
         free(buf);
     }
 }
 
 void orc_chandrupatla_f32(const float *inc, int64_t B, int64_t m, int order, const float *target,
-                          const float *knorm, int maxit, double tol, float *out, int *iters)
+                          const float *knorm, int maxit, double tol, float *out, int *iters,
+                          int *fixed)
 {
 #pragma omp parallel
     {
@@ -405,7 +416,7 @@ void orc_chandrupatla_f32(const float *inc, int64_t B, int64_t m, int order, con
         for (int64_t b = 0; b < B; ++b)
             out[b] = (float)chandrupatla_one(inc + b * m * m, m, order, (double)target[b],
                                              (double)knorm[b], 1, maxit, tol, buf,
-                                             iters ? iters + b : NULL);
+                                             iters ? iters + b : NULL, fixed ? fixed + b : NULL);
         free(buf);
     }
 }

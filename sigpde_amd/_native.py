@@ -41,6 +41,9 @@ _SIGNATURES = {
                            _vp, _int, _vp, _i64, _vp, ctypes.c_size_t, _vp],
     "sigpde_linear_pairwise": [_int, _int, _vp, _vp, _i64, _i64, _i64, _i64, _dbl, _int, _vp,
                                _vp, _vp, _vp, ctypes.c_size_t, _vp],
+    "sigpde_linear_gram_batch": [_int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _dbl, _int,
+                                 ctypes.POINTER(_i64), _i64, _vp, _vp, _vp, _vp, ctypes.c_size_t,
+                                 _vp],
     "sigpde_rbf_increments": [_int, _vp, _i64, _i64, _i64, _i64, ctypes.POINTER(_i64), _vp, _i64,
                               _vp, _i64, _i64, _dbl, _int, _vp, _vp],
     "sigpde_l1rbf_increments": [_int, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _int, _dbl, _int,
@@ -212,6 +215,38 @@ def linear_gram(X: torch.Tensor, Y: torch.Tensor, order: int, kscale: float, out
                               float(kscale), int(order), _ptr(sx), _ptr(sy), int(symmetric),
                               _ptr(out), out.stride(0), _ptr(ws), nbytes, _stream(X.device))
     _check(rc, "sigpde_linear_gram")
+
+
+MAX_GRAM_BATCH = 64
+
+
+def linear_gram_batch(X: torch.Tensor, Y: torch.Tensor, requests, order: int, kscale: float,
+                      out: torch.Tensor, precision: str = "fp16x2", sx=None, sy=None) -> None:
+    """Several fused linear-kernel Grams of one shape class in one launch. requests: up to 64
+    (x0, bx, y0, by, symmetric) rows over the paths of X and Y; out: a flat tensor of
+    sum(bx * by) elements receiving each request's row-major (bx, by) result in turn."""
+    _require_gpu(X, Y, out, sx, sy)
+    _require(X.dim() == 3 and Y.dim() == 3 and X.is_contiguous() and Y.is_contiguous()
+             and X.shape[2] == Y.shape[2],
+             "linear_gram_batch: X and Y must be contiguous (batch, length, dim) tensors")
+    reqs = [tuple(int(v) for v in q) for q in requests]
+    _require(len(reqs) <= MAX_GRAM_BATCH, f"linear_gram_batch: at most {MAX_GRAM_BATCH} requests")
+    total = sum(q[1] * q[3] for q in reqs)
+    _require(out.is_contiguous() and out.numel() == total,
+             f"linear_gram_batch: out must be contiguous with {total} elements")
+    _require_vec(sx, X.shape[0], "linear_gram_batch: sx")
+    _require_vec(sy, Y.shape[0], "linear_gram_batch: sy")
+    nx, Lx, d = X.shape
+    ny, Ly, _ = Y.shape
+    L = load()
+    prec = PRECISIONS[precision]
+    nbytes = L.sigpde_linear_workspace_bytes(prec, nx, Lx, ny, Ly, d, 0)
+    ws = _workspace(nbytes, X.device)
+    table = (_i64 * (5 * max(len(reqs), 1)))(*[v for q in reqs for v in q])
+    rc = L.sigpde_linear_gram_batch(dtype_code(X.dtype), prec, _ptr(X), nx, Lx, _ptr(Y), ny, Ly, d,
+                                    float(kscale), int(order), table, len(reqs), _ptr(sx), _ptr(sy),
+                                    _ptr(out), _ptr(ws), nbytes, _stream(X.device))
+    _check(rc, "sigpde_linear_gram_batch")
 
 
 def linear_pairwise(X: torch.Tensor, Y: torch.Tensor, order: int, kscale: float,

@@ -28,6 +28,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <climits>
+
 #include "capi_util.h"
 #include "mfma_producer.h"
 
@@ -115,38 +117,28 @@ __global__ __launch_bounds__(256) void k_split_f64(const T *__restrict__ X, int 
 // Gram: a wave takes four x paths (a quad) and streams `ys` y paths through them.
 // Symmetric: only y <= x pairs are written, with their mirror.
 
-// Occupancy: at dim <= 32 (one K block) the fp16x2 kernel fits three waves per SIMD (measured 9%
-// faster at dim 16); the other instantiations would spill there and keep the compiler's choice.
-// GL: boundary lines in global memory at gl (one block per wave of the launch), else in LDS.
+// One work unit of a fused Gram: x paths xb + 4 xq + g (g < 4, below xe) against the ys y paths
+// from yb + yr ys (below ye); the result of x path i and y path j goes to
+// out[(i - xb) * ldo + (j - yb)]. symmetric: the unit's x and y paths are the same paths (X == Y
+// over the unit's request), only y <= x pairs are solved and each value is also written to its
+// mirror.
 template <typename TO, int KB, int PREC, int R, bool GL>
-__global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(KB == 1 && PREC == kPrecFp16x2 && R == 1 ? 3 : 1))) void
-k_linear_gram_fused(GramFusedArgs<TO> a, const double *sxd, const double *syd, void *gl) {
+__device__ __forceinline__ void gram_unit(const GramFusedArgs<TO> &a, const double *sxd,
+                                          const double *syd, void *gl, int64_t xb, int64_t xe,
+                                          int64_t xq, int64_t yb, int64_t ye, int64_t yr,
+                                          bool symmetric, TO *out, int64_t ldo) {
   using Prod = MfmaProducer<KB, 4, 1, PREC>;
   using E = typename Prod::elt;
   constexpr Arith A = FusedArith<PREC>::A;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int G = 16;
   const int lane = threadIdx.x & 63;
-  const int64_t nq = (a.Bx + 3) / 4;
-  const int64_t nyr = (a.By + a.ys - 1) / a.ys;
-  // XCD-aware order: blocks are dealt round-robin to the 8 XCDs; each XCD takes a contiguous run
-  // of work units, ordered in super-tiles of tx x-quads by ty y-ranges, so that the waves resident
-  // on one XCD at a time share their x and y operands in that XCD's L2.
-  const int64_t bid = blockIdx.x;             // grid = 8 * per_xcd blocks
-  const int64_t per_xcd = gridDim.x / 8;
-  const int64_t lin = (bid % 8) * per_xcd + bid / 8;
-  const int64_t tsz = (int64_t)a.tx * a.ty;
-  const int64_t tile = lin / tsz, w = lin - tile * tsz;
-  const int64_t ntx = (nq + a.tx - 1) / a.tx;
-  const int64_t xq = (tile % ntx) * a.tx + w % a.tx;
-  const int64_t yr = (tile / ntx) * a.ty + w / a.tx;
-  if (xq >= nq || yr >= nyr) return;
-  const int64_t y0 = yr * a.ys;
-  int ys = (int)((a.By - y0) < a.ys ? (a.By - y0) : a.ys);
-  if (a.symmetric) {
+  const int64_t bx = xe - xb, by = ye - yb;
+  const int64_t y0 = yr * a.ys;  // local
+  int ys = (int)((by - y0) < a.ys ? (by - y0) : a.ys);
+  if (symmetric) {
     // only y <= max x of the quad
-    const int64_t xmax = xq * 4 + 3 < a.Bx - 1 ? xq * 4 + 3 : a.Bx - 1;
+    const int64_t xmax = xq * 4 + 3 < bx - 1 ? xq * 4 + 3 : bx - 1;
     if (y0 > xmax) return;
     if (y0 + ys - 1 > xmax) ys = (int)(xmax - y0 + 1);
   }
@@ -166,31 +158,95 @@ k_linear_gram_fused(GramFusedArgs<TO> a, const double *sxd, const double *syd, v
   P.sx = sxd;
   P.sy = syd;
   P.kmult = a.kmult;
-  P.ny = a.By;
+  P.ny = ye;
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
     const int64_t x = xq * 4 + g;
-    P.xa[g] = x < a.Bx ? x : -1;
+    P.xa[g] = x < bx ? xb + x : -1;
   }
-  P.ybase[0] = y0;
-  P.ylane = y0;
-  const int64_t xme = xq * 4 + lane / G;
+  P.ybase[0] = yb + y0;
+  P.ylane = yb + y0;
+  const int64_t xme = xq * 4 + lane / G;  // local
   {
-    const int64_t xc = xme < a.Bx ? xme : a.Bx - 1;
+    const int64_t xc = xb + (xme < bx ? xme : bx - 1);
     P.xsl = Prod::kSplit ? a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0) : 1.0;
     P.pmY = 1.0;
   }
   ring16_solve<A, R, GL>(P, geo, 1.0, smem_raw, [&](int sp, double r) {
     const int64_t y = y0 + sp;
-    if (xme >= a.Bx || y >= a.By) return;
-    if (a.symmetric) {
+    if (xme >= bx || y >= by) return;
+    if (symmetric) {
       if (y > xme) return;
-      a.out[xme * a.ldo + y] = (TO)r;
-      a.out[y * a.ldo + xme] = (TO)r;
+      out[xme * ldo + y] = (TO)r;
+      out[y * ldo + xme] = (TO)r;
     } else {
-      a.out[xme * a.ldo + y] = (TO)r;
+      out[xme * ldo + y] = (TO)r;
     }
   }, fused_wave_lines<A, GL>(gl, geo.N));
+}
+
+// Occupancy: at dim <= 32 (one K block) the fp16x2 kernel fits three waves per SIMD (measured 9%
+// faster at dim 16); the other instantiations would spill there and keep the compiler's choice.
+// GL: boundary lines in global memory at gl (one block per wave of the launch), else in LDS.
+template <typename TO, int KB, int PREC, int R, bool GL>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(KB == 1 && PREC == kPrecFp16x2 && R == 1 ? 3 : 1))) void
+k_linear_gram_fused(GramFusedArgs<TO> a, const double *sxd, const double *syd, void *gl) {
+  const int64_t nq = (a.Bx + 3) / 4;
+  const int64_t nyr = (a.By + a.ys - 1) / a.ys;
+  // XCD-aware order: blocks are dealt round-robin to the 8 XCDs; each XCD takes a contiguous run
+  // of work units, ordered in super-tiles of tx x-quads by ty y-ranges, so that the waves resident
+  // on one XCD at a time share their x and y operands in that XCD's L2.
+  const int64_t bid = blockIdx.x;             // grid = 8 * per_xcd blocks
+  const int64_t per_xcd = gridDim.x / 8;
+  const int64_t lin = (bid % 8) * per_xcd + bid / 8;
+  const int64_t tsz = (int64_t)a.tx * a.ty;
+  const int64_t tile = lin / tsz, w = lin - tile * tsz;
+  const int64_t ntx = (nq + a.tx - 1) / a.tx;
+  const int64_t xq = (tile % ntx) * a.tx + w % a.tx;
+  const int64_t yr = (tile / ntx) * a.ty + w / a.tx;
+  if (xq >= nq || yr >= nyr) return;
+  gram_unit<TO, KB, PREC, R, GL>(a, sxd, syd, gl, 0, a.Bx, xq, 0, a.By, yr, a.symmetric != 0,
+                                 a.out, a.ldo);
+}
+
+// Several Grams of one shape class in one launch: request r takes the units prefix[r] ..
+// prefix[r+1] - 1 (x-quads fastest), its x paths x0 .. x0 + bx - 1 and y paths y0 .. y0 + by - 1
+// of the launch's split operands, and writes its (bx, by) result at out + out_off, row-major.
+// The request table travels by value in the kernel arguments (no device table, nothing to
+// allocate; a captured launch holds it).
+constexpr int kMaxGramBatch = 64;
+struct GramBatchReq {
+  int32_t x0, bx, y0, by, prefix, symmetric;
+  int64_t out_off;
+};
+struct GramBatch {
+  int32_t n, nunits;
+  GramBatchReq r[kMaxGramBatch];
+};
+
+template <typename TO, int KB, int PREC, int R, bool GL>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(KB == 1 && PREC == kPrecFp16x2 && R == 1 ? 3 : 1))) void
+k_linear_gram_batch(GramFusedArgs<TO> a, const double *sxd, const double *syd, void *gl,
+                    GramBatch b) {
+  // blocks dealt round-robin to the 8 XCDs, each XCD a contiguous run of units (of few requests)
+  const int64_t bid = blockIdx.x;
+  const int64_t per_xcd = gridDim.x / 8;
+  const int64_t lin = (bid % 8) * per_xcd + bid / 8;
+  if (lin >= b.nunits) return;
+  int lo = 0, hi = b.n - 1;  // the last request whose first unit is <= lin
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (b.r[mid].prefix <= lin) lo = mid;
+    else hi = mid - 1;
+  }
+  const GramBatchReq q = b.r[lo];
+  const int64_t u = lin - q.prefix;
+  const int64_t nq = (q.bx + 3) / 4;
+  gram_unit<TO, KB, PREC, R, GL>(a, sxd, syd, gl, q.x0, (int64_t)q.x0 + q.bx, u % nq, q.y0,
+                                 (int64_t)q.y0 + q.by, u / nq, q.symmetric != 0,
+                                 a.out + q.out_off, q.by);
 }
 
 // double copies of optional scales
@@ -273,11 +329,14 @@ static hipError_t run_to_double(const void *src, int64_t n, double *dst, hipStre
   return hipGetLastError();
 }
 
+// batch (Gram only, may be null): several Grams over the Bx x paths and By y paths (see
+// k_linear_gram_batch); its units and prefixes are filled in here, once the stream length is set.
 template <typename T, int KB, int PREC>
 static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int64_t Bx, int64_t By,
                                int64_t Lx, int64_t Ly, int64_t d, double kscale, int order,
                                const void *sx, const void *sy, int symmetric, void *out,
-                               int64_t ldo, const Workspace &w, hipStream_t st) {
+                               int64_t ldo, const Workspace &w, const GramBatch *batch,
+                               hipStream_t st) {
   const int dp = dpad_for(d);
   hipError_t e = run_split<T, PREC>((const T *)X, Bx, Lx, d, dp, w.xh, w.xm, w.xl, w.invx, st);
   if (e != hipSuccess) return e;
@@ -314,8 +373,15 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
     // y paths streamed per wave: enough waves to fill the GPU, long enough streams to amortise
     // the 15-step pipeline fill of each pair
     const int64_t nq = (Bx + 3) / 4;
+    auto units = [&](int ysv) -> int64_t {
+      if (!batch) return nq * ((By + ysv - 1) / ysv);
+      int64_t u = 0;
+      for (int r = 0; r < batch->n; ++r)
+        u += (int64_t)((batch->r[r].bx + 3) / 4) * ((batch->r[r].by + ysv - 1) / ysv);
+      return u;
+    };
     int ys = 16;
-    while (ys > 1 && nq * ((By + ys - 1) / ys) < 4096) ys >>= 1;
+    while (ys > 1 && units(ys) < 4096) ys >>= 1;
     // a wave's virtual-column count (ys pairs x bands x band length) must stay within int
     {
       const PairGeom<16> g1(a.m, a.n, order, 1, r16_rows(order));
@@ -326,7 +392,18 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
     // super-tiles of about one XCD's resident waves
     a.tx = (int)(nq < kTileX ? nq : kTileX);
     a.ty = (int)(nyr < kTileY ? nyr : kTileY);
-    const int64_t nblk = (nq + a.tx - 1) / a.tx * a.tx * ((nyr + a.ty - 1) / a.ty * a.ty);
+    GramBatch gb{};
+    int64_t nblk = (nq + a.tx - 1) / a.tx * a.tx * ((nyr + a.ty - 1) / a.ty * a.ty);
+    if (batch) {
+      gb = *batch;
+      int64_t pre = 0;
+      for (int r = 0; r < gb.n; ++r) {
+        gb.r[r].prefix = (int32_t)pre;
+        pre += (int64_t)((gb.r[r].bx + 3) / 4) * ((gb.r[r].by + ys - 1) / ys);
+      }
+      gb.nunits = (int32_t)pre;
+      nblk = pre;
+    }
     // a multiple of 8 blocks so that the XCD-major block order covers every work unit
     const int64_t grid = (nblk + 7) / 8 * 8;
     const dim3 g3((unsigned)grid), b3(64);
@@ -338,10 +415,18 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
           reinterpret_cast<const void *>(&k_linear_gram_fused<T, KB, PREC, R, false>),
           reinterpret_cast<const void *>(&k_linear_gram_fused<T, KB, PREC, R, true>), N, grid, st);
       return with_fused_lines<A>(gl, N, grid, st, [&](void *lines, size_t shmem) {
-        if (lines)
+        if (batch) {
+          if (lines)
+            k_linear_gram_batch<T, KB, PREC, R, true><<<g3, b3, shmem, st>>>(a, sxd, syd, lines,
+                                                                               gb);
+          else
+            k_linear_gram_batch<T, KB, PREC, R, false><<<g3, b3, shmem, st>>>(a, sxd, syd,
+                                                                                nullptr, gb);
+        } else if (lines) {
           k_linear_gram_fused<T, KB, PREC, R, true><<<g3, b3, shmem, st>>>(a, sxd, syd, lines);
-        else
+        } else {
           k_linear_gram_fused<T, KB, PREC, R, false><<<g3, b3, shmem, st>>>(a, sxd, syd, nullptr);
+        }
       });
     };
     switch (r16_rows(order)) {
@@ -357,12 +442,13 @@ template <typename T, int PREC>
 static hipError_t dispatch_kb(bool pairwise, const void *X, const void *Y, int64_t Bx, int64_t By,
                               int64_t Lx, int64_t Ly, int64_t d, double kscale, int order,
                               const void *sx, const void *sy, int symmetric, void *out,
-                              int64_t ldo, const Workspace &w, hipStream_t st) {
+                              int64_t ldo, const Workspace &w, const GramBatch *batch,
+                              hipStream_t st) {
   const int kb = dpad_for(d) / kKBlock;
 #define SIGPDE_KB_CASE(K)                                                                          \
   case K:                                                                                          \
     return launch_fused<T, K, PREC>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,       \
-                                    symmetric, out, ldo, w, st);
+                                    symmetric, out, ldo, w, batch, st);
   if constexpr (PREC == kPrecFp64) {  // dim <= 64
     switch (kb) {
       SIGPDE_KB_CASE(1)
@@ -388,18 +474,18 @@ static hipError_t dispatch_prec(int precision, bool pairwise, const void *X, con
                                 int64_t Bx, int64_t By, int64_t Lx, int64_t Ly, int64_t d,
                                 double kscale, int order, const void *sx, const void *sy,
                                 int symmetric, void *out, int64_t ldo, const Workspace &w,
-                                hipStream_t st) {
+                                const GramBatch *batch, hipStream_t st) {
   if (precision == SIGPDE_PREC_FP32)
     return dispatch_kb<T, kPrecFp32>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
-                                     symmetric, out, ldo, w, st);
+                                     symmetric, out, ldo, w, batch, st);
   if (precision == SIGPDE_PREC_BF16X3)
     return dispatch_kb<T, kPrecBf16x3>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
-                                       symmetric, out, ldo, w, st);
+                                       symmetric, out, ldo, w, batch, st);
   if (precision == SIGPDE_PREC_FP64)
     return dispatch_kb<T, kPrecFp64>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
-                                     symmetric, out, ldo, w, st);
+                                     symmetric, out, ldo, w, batch, st);
   return dispatch_kb<T, kPrecFp16x2>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
-                                     symmetric, out, ldo, w, st);
+                                     symmetric, out, ldo, w, batch, st);
 }
 
 constexpr int64_t kMaxDim = 4 * kKBlock;
@@ -481,10 +567,67 @@ extern "C" int sigpde_linear_gram(int dtype, int precision, const void *X, int64
   hipStream_t st = (hipStream_t)stream;
   hipError_t e = dtype == SIGPDE_F32
                      ? dispatch_prec<float>(precision, false, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
-                                          symmetric, out, ldo, w, st)
+                                          symmetric, out, ldo, w, nullptr, st)
                      : dispatch_prec<double>(precision, false, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
-                                           symmetric, out, ldo, w, st);
+                                           symmetric, out, ldo, w, nullptr, st);
   return hip_status(e, "linear_gram launch");
+}
+
+extern "C" int sigpde_linear_gram_batch(int dtype, int precision, const void *X, int64_t nx,
+                                        int64_t Lx, const void *Y, int64_t ny, int64_t Ly,
+                                        int64_t d, double kscale, int order,
+                                        const int64_t *requests, int64_t nreq, const void *sx,
+                                        const void *sy, void *out, void *workspace,
+                                        size_t ws_bytes, void *stream) {
+  clear_error();
+  if (nreq < 0 || nreq > kMaxGramBatch)
+    return set_error(SIGPDE_EINVAL, "linear_gram_batch: %lld requests, at most %d per launch",
+                     (long long)nreq, kMaxGramBatch);
+  if (nreq > 0 && !requests) return set_error(SIGPDE_EINVAL, "linear_gram_batch: null request table");
+  GramBatch gb{};
+  int64_t out_elems = 0;
+  for (int64_t r = 0; r < nreq; ++r) {
+    const int64_t *q = requests + 5 * r;
+    const int64_t x0 = q[0], bx = q[1], y0 = q[2], by = q[3], sym = q[4];
+    if (bx < 0 || by < 0 || x0 < 0 || y0 < 0 || x0 + bx > nx || y0 + by > ny)
+      return set_error(SIGPDE_EINVAL, "linear_gram_batch: request %lld out of the %lld x / %lld y "
+                       "paths", (long long)r, (long long)nx, (long long)ny);
+    if (sym && (bx != by || Lx != Ly))
+      return set_error(SIGPDE_EINVAL, "linear_gram_batch: symmetric request %lld needs bx == by "
+                       "and Lx == Ly", (long long)r);
+    if (bx == 0 || by == 0) continue;  // empty result: no units
+    GramBatchReq &g = gb.r[gb.n++];
+    g.x0 = (int32_t)x0;
+    g.bx = (int32_t)bx;
+    g.y0 = (int32_t)y0;
+    g.by = (int32_t)by;
+    g.symmetric = sym ? 1 : 0;
+    g.out_off = out_elems;
+    out_elems += bx * by;
+  }
+  if (nx > INT32_MAX || ny > INT32_MAX)
+    return set_error(SIGPDE_EINVAL, "linear_gram_batch: more than 2^31 paths");
+  {
+    int64_t units = 0;  // at the longest y stream (one y path a wave): an upper bound
+    for (int r = 0; r < gb.n; ++r) units += (int64_t)((gb.r[r].bx + 3) / 4) * gb.r[r].by;
+    if (units > INT32_MAX)
+      return set_error(SIGPDE_EINVAL, "linear_gram_batch: more than 2^31 work units");
+  }
+  if (int rc = check_fused(dtype, precision, X, Y, nx, ny, Lx, Ly, d, order, out, workspace))
+    return rc;
+  if (gb.n == 0) return SIGPDE_OK;
+  Workspace w;
+  const size_t need = workspace_layout(precision, nx, Lx, ny, Ly, d, 0, (char *)workspace, &w);
+  if (ws_bytes < need)
+    return set_error(SIGPDE_EINVAL, "linear_gram_batch: workspace of %zu bytes, need %zu",
+                     ws_bytes, need);
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = dtype == SIGPDE_F32
+                     ? dispatch_prec<float>(precision, false, X, Y, nx, ny, Lx, Ly, d, kscale,
+                                            order, sx, sy, 0, out, 0, w, &gb, st)
+                     : dispatch_prec<double>(precision, false, X, Y, nx, ny, Lx, Ly, d, kscale,
+                                             order, sx, sy, 0, out, 0, w, &gb, st);
+  return hip_status(e, "linear_gram_batch launch");
 }
 
 extern "C" int sigpde_linear_pairwise(int dtype, int precision, const void *X, const void *Y,
@@ -503,8 +646,8 @@ extern "C" int sigpde_linear_pairwise(int dtype, int precision, const void *X, c
   hipStream_t st = (hipStream_t)stream;
   hipError_t e = dtype == SIGPDE_F32
                      ? dispatch_prec<float>(precision, true, X, Y, B, B, Lx, Ly, d, kscale, order, sx, sy, 0,
-                                          out, 1, w, st)
+                                          out, 1, w, nullptr, st)
                      : dispatch_prec<double>(precision, true, X, Y, B, B, Lx, Ly, d, kscale, order, sx, sy, 0,
-                                           out, 1, w, st);
+                                           out, 1, w, nullptr, st);
   return hip_status(e, "linear_pairwise launch");
 }

@@ -208,6 +208,11 @@ __device__ __forceinline__ double log_normalizer(double x) {
 // bracket end's |F| < tol (function value only) or after maxit evaluations. The bracket
 // arithmetic is not contracted into FMAs (every product and quotient rounded as written), so that
 // with the same function values the iterates, and lambda, equal the oracle's bit for bit.
+// It also stops at a fixed point: when an iteration leaves the whole state (a, b, c, fa, fb, fc,
+// t) unchanged, every later iteration repeats it (the wavefront is deterministic), so the result
+// is the one maxit evaluations would give, bit for bit. Float32 paths whose function values
+// cannot reach tol (most of them at tol 1e-8: F is quantised to float32) reach such a point after
+// about 40-50 evaluations instead of running all maxit.
 template <typename T>
 struct RootFind {
   double av, cv, bv, fa, fb, fc, t, outv;
@@ -232,6 +237,7 @@ struct RootFind {
   // Takes F's evaluation at xt = next() (the wavefront result sol); true when the search stops.
   __device__ bool update(double xt, double sol, double tol, int maxit) {
 #pragma clang fp contract(off)
+    const double a0 = av, b0 = bv, c0 = cv, fa0 = fa, fb0 = fb, fc0 = fc, t0 = t;
     ++it;
     T solT = (T)sol;
     double ft = isnan(solT) ? (double)INFINITY : (double)(T)(solT - Tt);
@@ -261,7 +267,9 @@ struct RootFind {
     else
       t = 0.5;
     if (!(0 < t && t < 1)) t = 0.5;
-    return stop || it >= maxit;
+    const bool fixed = av == a0 && bv == b0 && cv == c0 && fa == fa0 && fb == fb0 && fc == fc0 &&
+                       t == t0;
+    return stop || fixed || it >= maxit;
   }
 };
 

@@ -7,9 +7,12 @@ sigpde/torch/kernel.py:26-301 API) for a stream of independent requests:
 
   * Requests of one "shape class" (kind, static kernel, dyadic order, precision, dtype, dim,
     bucketed lengths, root-find settings) that are pair- or path-separable (pairwise,
-    normalization, robust pairwise) are concatenated into one launch; Gram requests run as
-    launches of their own. Every pair or path of a batch is solved independently, so a request's
-    result does not depend on what it was batched with.
+    normalization, robust pairwise) are concatenated into one launch. Gram and robust-Gram
+    requests with a fused precision (the linear kernel on MFMA) are coalesced too: up to 64 of
+    one class run as one launch of the batched fused Gram (sigpde_linear_gram_batch), their
+    robust root-finds as one normalisation launch over all their paths; reference-precision
+    Grams (increment tiles in HBM) run as launches of their own. Every pair or path of a batch is
+    solved independently, so a request's result does not depend on what it was batched with.
   * Length buckets: a path is padded to its bucket's length by repeating its last point. The
     appended increments are zero, where the PDE step reduces to k = k01 + k10 - k00 and the
     solution at the corner is unchanged (exactly in real arithmetic; to rounding in floating
@@ -17,11 +20,17 @@ sigpde/torch/kernel.py:26-301 API) for a stream of independent requests:
   * HBM budget: a batch is admitted only while the estimated bytes of the batches in flight
     (inputs, outputs, increment tiles or fused workspace) stay within the budget; a request that
     cannot fit alone fails with MemoryError.
-  * Streams: batches are issued round-robin on `streams` HIP streams of default priority;
-    completion is tracked with events.
-  * HIP graphs (use_graphs=True): fused pairwise batches (precision "fp16x2", "bf16x3", "fp32" or
-    "fp64") are captured once per (shape class, padded batch size) and replayed with the inputs
-    copied into the graph's static buffers.
+  * Streams: each batch goes to the one of `streams` HIP streams (default priority) with the least
+    estimated work in flight (ties round-robin); completion is tracked with events. Results are
+    recorded on the stream current when they are handed over (poll), so a caller's later use
+    keeps their memory alive.
+  * HIP graphs (use_graphs=True, the default): fused pairwise batches (precision "fp16x2",
+    "bf16x3", "fp32" or "fp64") are captured once per (shape class, padded batch size, stream)
+    and replayed with the inputs copied into that graph's static buffers (one set per stream:
+    batches of one class on two streams never share them).
+  * Deadlines and cancellation: a request with a `timeout` (seconds after submission) that is
+    still queued when it expires fails with TimeoutError without being launched; cancel(r)
+    removes a queued request (CancelledError), or drops the result of a launched one.
 
 `python -m sigpde_amd.serving` runs a synthetic request mix and prints one JSON line.
 """
@@ -65,12 +74,14 @@ class Request:
     tol: float = 1e-8
     maxit: int = 100
     arrival: float = 0.0          # seconds after the server's start (synthetic arrival time)
+    timeout: float = math.inf     # seconds after submission; a request still queued then fails
     # filled by the server
     id: int = -1
     t_submit: float = math.nan
     t_done: float = math.nan
     result: Optional[torch.Tensor] = None
     error: Optional[BaseException] = None
+    cancelled: bool = False
 
     def __post_init__(self):
         if self.kind not in KINDS:
@@ -117,6 +128,17 @@ def pad_paths(x: torch.Tensor, L: int) -> torch.Tensor:
     return torch.cat([x, tail], dim=1)
 
 
+class CancelledError(Exception):
+    """The request was cancelled (Server.cancel) before its result was handed over."""
+
+
+def gram_coalesced(r: Request) -> bool:
+    """Gram kinds whose requests of one class share a launch: fused precisions (the linear static
+    kernel on the MFMA producer)."""
+    return (r.kind in ("gram", "robust_gram") and r.precision != "ref"
+            and getattr(r.static_kernel, "fused_linear_scale", None) is not None)
+
+
 def batch_key(r: Request, buckets) -> Tuple:
     """Requests with equal keys may share one launch."""
     Lx = bucket_length(r.x.shape[1], buckets)
@@ -151,10 +173,12 @@ def estimate_bytes(kind: str, precision: str, B: int, By: int, Lx: int, Ly: int,
     chunk = min(B, max_batch)
     if precision == "ref":
         inc = chunk * m * n * itemsize * 2
-        if kind != "pairwise":
-            inc += 2 * chunk * (m * m + n * n) * itemsize
     else:
         inc = _native.linear_workspace_bytes(precision, B, Lx, B, Ly, d, False)
+    if kind != "pairwise":
+        # the root-find's self-increments of x and y (or of x and y joined), whatever precision
+        # the final solves take
+        inc += 2 * chunk * (m * m + n * n) * itemsize
     return inputs + out + inc
 
 
@@ -179,16 +203,19 @@ class Server:
     """Continuous-batching server for one GPU.
 
     hbm_budget: bytes of device memory the in-flight launches may hold (default 64 GiB);
-    max_pairs: pairs or paths per separable batch; streams: HIP streams (default priority);
-    bucket_lengths: length buckets for separable batches (None: exact lengths only);
-    max_batch: the API's max_batch (increment tile size of precision "ref")."""
+    max_pairs: pairs or paths per separable batch; max_gram_pairs: output pairs per coalesced
+    Gram batch (at most 64 requests); streams: HIP streams (default priority); bucket_lengths:
+    length buckets for separable and coalesced Gram batches (None: exact lengths only);
+    max_batch: the API's max_batch (increment tile size of precision "ref"); use_graphs: replay
+    fused pairwise batches through captured HIP graphs."""
 
     def __init__(self, device="cuda", hbm_budget: int = 64 << 30, max_pairs: int = 16384,
                  streams: int = 4, bucket_lengths=DEFAULT_BUCKETS, max_batch: int = 4096,
-                 use_graphs: bool = False):
+                 use_graphs: bool = True, max_gram_pairs: int = 1 << 22):
         self.device = torch.device(device)
         self.hbm_budget = int(hbm_budget)
         self.max_pairs = int(max_pairs)
+        self.max_gram_pairs = int(max_gram_pairs)
         self.buckets = tuple(bucket_lengths) if bucket_lengths else None
         self.max_batch = int(max_batch)
         self.use_graphs = use_graphs
@@ -212,9 +239,37 @@ class Server:
         r.id = self._next_id
         self._next_id += 1
         r.t_submit = self.now()
-        key = batch_key(r, self.buckets if r.kind in SEPARABLE else None)
+        key = batch_key(r, self.buckets if (r.kind in SEPARABLE or gram_coalesced(r)) else None)
         self._queue.setdefault(key, []).append(r)
         return r
+
+    def cancel(self, r: Request) -> bool:
+        """Cancels r: a queued request is removed and fails with CancelledError (True); a launched
+        one still runs, and its result is dropped when its batch completes (False)."""
+        r.cancelled = True
+        for key, reqs in list(self._queue.items()):
+            if any(q is r for q in reqs):
+                self._remove(key, [r])
+                self._fail(r, CancelledError(f"request {r.id} cancelled"))
+                return True
+        return False
+
+    def _fail(self, r: Request, e: BaseException) -> None:
+        r.error = e
+        r.result = None
+        r.t_done = self.now()
+        self.completed.append(r)
+
+    def _expire(self) -> None:
+        """Fails the queued requests whose deadline has passed."""
+        now = self.now()
+        for key, reqs in list(self._queue.items()):
+            late = [r for r in reqs if now - r.t_submit > r.timeout]
+            if late:
+                self._remove(key, late)
+                for r in late:
+                    self._fail(r, TimeoutError(f"request {r.id} still queued after its timeout "
+                                               f"of {r.timeout} s"))
 
     # ------------------------------------------------------------------ planning
     def _request_bytes(self, key, reqs: List[Request]) -> int:
@@ -225,6 +280,15 @@ class Server:
             Lyy = Lx if Ly < 0 else Ly
             return estimate_bytes(kind, precision, B, B, Lx, Lyy, d, order, itemsize,
                                   self.max_batch)
+        if gram_coalesced(reqs[0]):
+            # the joined paths of the batch: every request's x and y paths, one workspace
+            Bx = sum(r.x.shape[0] for r in reqs)
+            By = sum((r.x if r.y is None else r.y).shape[0] for r in reqs)
+            Lyy = Lx if Ly < 0 else Ly
+            n = estimate_bytes(kind, precision, Bx, By, Lx, Lyy, d, order, itemsize,
+                               self.max_batch)
+            # the outputs are per request, not Bx x By
+            return n - Bx * By * itemsize + sum(r.pairs() for r in reqs) * itemsize
         r = reqs[0]
         By = r.x.shape[0] if r.y is None else r.y.shape[0]
         Lyy = r.x.shape[1] if r.y is None else r.y.shape[1]
@@ -244,6 +308,14 @@ class Server:
                         break
                     take.append(r)
                     pairs += b
+            elif gram_coalesced(reqs[0]):
+                take, pairs = [], 0
+                for r in reqs:
+                    if take and (len(take) >= _native.MAX_GRAM_BATCH
+                                 or pairs + r.pairs() > self.max_gram_pairs):
+                        break
+                    take.append(r)
+                    pairs += r.pairs()
             else:
                 take = reqs[:1]
             nbytes = self._request_bytes(key, take)
@@ -308,6 +380,8 @@ class Server:
         kind, _, order, precision, dtype, d, Lx, Ly = b.key[:8]
         r0 = b.reqs[0]
         k = r0.static_kernel
+        if gram_coalesced(r0):
+            return self._gram_batch(b)
         if kind == "gram":
             return [SigPDE(k, order, precision).gram(r0.x, r0.y, max_batch=self.max_batch)]
         if kind == "robust_gram":
@@ -336,13 +410,53 @@ class Server:
                 out = rk.pairwise(X, Y, tol=r0.tol, maxit=r0.maxit, max_batch=self.max_batch)
         return list(torch.split(out, sizes))
 
+    def _gram_batch(self, b: _Batch) -> List[torch.Tensor]:
+        """A coalesced batch of fused Gram / robust-Gram requests of one class: their paths
+        joined (padded to the bucket lengths), the robust root-finds of all of them in one
+        normalisation call, then one launch of the batched fused Gram."""
+        kind, _, order, precision, dtype, d, Lx, Ly = b.key[:8]
+        r0 = b.reqs[0]
+        k = r0.static_kernel
+        symmetric = Ly < 0
+        X = torch.cat([pad_paths(r.x, Lx) for r in b.reqs])
+        Y = X if symmetric else torch.cat([pad_paths(r.y, Ly) for r in b.reqs])
+        rows, x0, y0 = [], 0, 0
+        for r in b.reqs:
+            bx = r.x.shape[0]
+            by = bx if symmetric else r.y.shape[0]
+            rows.append((x0, bx, x0 if symmetric else y0, by, int(symmetric)))
+            x0 += bx
+            y0 += by
+        sx = sy = None
+        if kind == "robust_gram":
+            rk = RobustSigPDE(k, order, r0.normalizer)
+            if symmetric:
+                sx = sy = rk.normalization(X, tol=r0.tol, maxit=r0.maxit,
+                                           max_batch=self.max_batch)
+            elif Lx == Ly:
+                lam = rk.normalization(torch.cat([X, Y]), tol=r0.tol, maxit=r0.maxit,
+                                       max_batch=self.max_batch)
+                sx, sy = lam[:X.shape[0]].contiguous(), lam[X.shape[0]:].contiguous()
+            else:
+                sx = rk.normalization(X, tol=r0.tol, maxit=r0.maxit, max_batch=self.max_batch)
+                sy = rk.normalization(Y, tol=r0.tol, maxit=r0.maxit, max_batch=self.max_batch)
+        out = torch.empty(sum(q[1] * q[3] for q in rows), device=X.device, dtype=dtype)
+        _native.linear_gram_batch(X, Y, rows, order, float(k.fused_linear_scale), out, precision,
+                                  sx, sy)
+        res, off = [], 0
+        for q in rows:
+            res.append(out[off:off + q[1] * q[3]].view(q[1], q[3]))
+            off += q[1] * q[3]
+        return res
+
     def _pairwise_graph(self, key, k, order, precision, X, Y) -> torch.Tensor:
-        """Fused pairwise through a captured HIP graph (one per shape class and padded size)."""
+        """Fused pairwise through a captured HIP graph (one per shape class, padded size and
+        stream: the static buffers of one graph are never shared by two streams)."""
         B = X.shape[0]
         Bp = max(4, 1 << (B - 1).bit_length())
-        gk = (key, Bp)
-        g = self._graphs.get(gk)
         stream = torch.cuda.current_stream(self.device)
+        gk = (key, Bp, stream.cuda_stream)
+        g = self._graphs.get(gk)
         if g is None:
             g = _GraphEntry()
             g.X = torch.zeros((Bp,) + tuple(X.shape[1:]), device=X.device, dtype=X.dtype)
@@ -370,7 +484,14 @@ class Server:
                 b.event.synchronize()
             if block or b.event.query():
                 t = self.now()
+                cur = torch.cuda.current_stream(self.device)
                 for r, o in zip(b.reqs, b.outputs):
+                    if r.cancelled and r.error is None:
+                        r.error = CancelledError(f"request {r.id} cancelled")
+                        o = None
+                    if o is not None:
+                        # made on the batch's stream: keep its memory for the caller's stream
+                        o.record_stream(cur)
                     r.result = o
                     r.t_done = t
                     self.completed.append(r)
@@ -383,6 +504,7 @@ class Server:
 
     def step(self) -> int:
         """Launches every batch that fits now; returns how many."""
+        self._expire()
         n = 0
         while self._queue:
             b = self._take_batch()

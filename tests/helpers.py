@@ -32,3 +32,12 @@ def quantized_paths(batch, length, dim, seed, bits=6, dtype=np.float32):
     x = np.zeros((batch, length, dim), dtype=np.float64)
     x[:, 1:, :] = np.cumsum(steps, axis=1) * 2.0 ** (-bits)
     return x.astype(dtype)
+
+
+def evals_with_fixed_point_exit(iters, fixed):
+    """Wavefront evaluations of a root-find that also stops at a fixed point of its state: the
+    oracle's fixed point where it has one (the reference keeps iterating from there without
+    changing lambda), else the oracle's own count."""
+    iters = np.asarray(iters)
+    fixed = np.asarray(fixed)
+    return np.where(fixed >= 0, fixed, iters).astype(iters.dtype)

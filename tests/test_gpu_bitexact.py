@@ -17,7 +17,7 @@ import torch
 
 import oracle as O
 import sigpde_amd.torch as sig
-from helpers import quantized_paths, rel_fro
+from helpers import evals_with_fixed_point_exit, quantized_paths, rel_fro
 from sigpde_amd import _native
 
 pytestmark = pytest.mark.gpu
@@ -105,7 +105,8 @@ def test_gram_tile_routes(dtype, bx, by, m, n, order, symmetric):
 @pytest.mark.parametrize("B,m,order", [(16, 31, 1), (2050, 12, 1), (3, 300, 0), (2, 150, 2)])
 def test_root_find_given_target(dtype, B, m, order):
     """K(x,x) equal bit for bit; with the same target (a custom normaliser computed on the host),
-    every iterate, the evaluation count and lambda equal the oracle's bit for bit."""
+    every iterate and lambda equal the oracle's bit for bit, and the evaluation count is the
+    oracle's, or its first fixed point of the iteration, where the device stops."""
     rng = np.random.default_rng(B + m)
     dx = rng.normal(size=(B, m, 5)) * (1.2 / np.sqrt(m))
     inc = (np.einsum("bsk,btk->bst", dx, dx) / 4.0 ** order).astype(NP[dtype])
@@ -119,10 +120,15 @@ def test_root_find_given_target(dtype, B, m, order):
     lam = torch.zeros(B, dtype=dtype, device=DEV)
     iters = torch.zeros(B, dtype=torch.int32, device=DEV)
     _native.robust_norm(gpu(inc), order, lam, kn, gpu(target), maxit=60, tol=1e-10, iters=iters)
-    lam_ref, it_ref = O.chandrupatla(inc, order, target, k, 60, 1e-10)
+    lam_ref, it_ref, fx_ref = O.chandrupatla(inc, order, target, k, 60, 1e-10, fixed_points=True)
     lam_ref[k < 1] = 0
     same(lam.cpu().numpy(), lam_ref)
-    same(iters.cpu().numpy(), it_ref)
+    # the device stops at a fixed point of the iteration (lambda unchanged); the oracle runs on
+    same(iters.cpu().numpy(), evals_with_fixed_point_exit(it_ref, fx_ref))
+    if dtype == torch.float32:
+        # float32 function values cannot reach tol 1e-10: paths end at a fixed point, and lambda
+        # is still the oracle's 60-evaluation result bit for bit
+        assert (fx_ref >= 0).any() and (fx_ref < 60).any()
 
 
 @pytest.mark.parametrize("case", [

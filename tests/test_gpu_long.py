@@ -11,7 +11,7 @@ import torch
 
 import oracle as O
 import sigpde_amd.torch as sig
-from helpers import brownian_np, max_rel, rel_fro
+from helpers import brownian_np, max_rel, rel_fro, evals_with_fixed_point_exit
 from sigpde_amd import _native
 
 pytestmark = pytest.mark.gpu
@@ -81,7 +81,7 @@ def test_robust_norm_global_lines():
     rng = np.random.default_rng(13)
     dx = rng.normal(size=(B, m, 2)) * (1.3 / np.sqrt(m))
     inc = np.einsum("bsk,btk->bst", dx, dx) / 4.0 ** order
-    lam_ref, k_ref, it_ref = O.norm_factors(inc, order, tol=1e-10, maxit=2)
+    lam_ref, k_ref, it_ref, fx_ref = O.norm_factors(inc, order, tol=1e-10, maxit=2, fixed_points=True)
     lam = torch.zeros(B, dtype=torch.float64, device=DEV)
     kn = torch.zeros(B, dtype=torch.float64, device=DEV)
     iters = torch.zeros(B, dtype=torch.int32, device=DEV)
@@ -89,7 +89,7 @@ def test_robust_norm_global_lines():
                         iters=iters)
     assert rel_fro(kn.cpu().numpy(), k_ref) < 1e-10
     assert max_rel(lam.cpu().numpy(), lam_ref, floor=1e-3) < 1e-9
-    assert np.array_equal(iters.cpu().numpy(), it_ref)
+    assert np.array_equal(iters.cpu().numpy(), evals_with_fixed_point_exit(it_ref, fx_ref))
 
 
 def test_grid_too_large_raises():

@@ -11,7 +11,7 @@ import pytest
 import torch
 
 import oracle as O
-from helpers import max_rel, rel_fro
+from helpers import max_rel, rel_fro, evals_with_fixed_point_exit
 from sigpde_amd import _native
 
 pytestmark = pytest.mark.gpu
@@ -102,7 +102,7 @@ def test_robust_norm_many(dtype):
     dx = rng.normal(size=(B, m, 3)) * 0.45
     inc64 = np.einsum("bsk,btk->bst", dx, dx) / 4.0 ** order
     inc = inc64.astype(NP[dtype])
-    lam_ref, k_ref, it_ref = O.norm_factors(inc, order, tol=1e-10, maxit=100)
+    lam_ref, k_ref, it_ref, fx_ref = O.norm_factors(inc, order, tol=1e-10, maxit=100, fixed_points=True)
     lam = torch.zeros(B, dtype=dtype, device=DEV)
     kn = torch.zeros(B, dtype=dtype, device=DEV)
     iters = torch.zeros(B, dtype=torch.int32, device=DEV)
@@ -112,4 +112,4 @@ def test_robust_norm_many(dtype):
     tol_lam = 1e-9 if dtype == torch.float64 else 1e-5
     assert max_rel(lam.cpu().numpy(), lam_ref, floor=1e-3) < tol_lam
     if dtype == torch.float64:
-        assert np.array_equal(iters.cpu().numpy(), it_ref)
+        assert np.array_equal(iters.cpu().numpy(), evals_with_fixed_point_exit(it_ref, fx_ref))

@@ -213,3 +213,93 @@ def test_empty_requests_complete_with_empty_results():
         assert r.error is None, (r.kind, r.error)
         assert r.result.shape == a.shape, r.kind
         torch.testing.assert_close(r.result, a, rtol=2e-4, atol=1e-6)
+
+
+@pytest.mark.parametrize("prec", ["fp16x2", "fp32"])
+def test_graph_replays_on_several_streams_equal_eager(prec):
+    """Several batches of one shape class in flight on three streams at once: each stream replays
+    its own captured graph with its own static buffers, and every result equals the eager call."""
+    k = sig.kernels.LinearKernel()
+    srv = S.Server(DEV, use_graphs=True, streams=3, max_pairs=3)
+    reqs = [S.Request("pairwise", quantized(3, 17, 4, 500 + i), quantized(3, 17, 4, 600 + i),
+                      dyadic_order=1, precision=prec) for i in range(12)]
+    srv.run(reqs, realtime=False)
+    torch.cuda.synchronize()
+    assert srv.launches == 12 and len(srv._graphs) == 3  # one graph per stream
+    eager = [sig.SigPDE(k, 1, prec).pairwise(r.x, r.y) for r in reqs]
+    torch.cuda.synchronize()
+    for r, e in zip(reqs, eager):
+        assert r.error is None, r.error
+        assert torch.equal(r.result, e), r.id
+
+
+def test_fused_gram_requests_share_a_launch_and_equal_alone():
+    """Fused Gram and robust-Gram requests of one class are served by one batched launch each,
+    and every result equals the direct API call bit for bit (exact lengths)."""
+    reqs = []
+    for i in range(6):
+        x = quantized(3 + i, 17, 4, 700 + i)
+        y = None if i % 2 else quantized(2 + i, 17, 4, 800 + i)
+        kind = "robust_gram" if i >= 4 else "gram"
+        reqs.append(S.Request(kind, x, y, dyadic_order=1, precision="fp16x2", tol=1e-6,
+                              maxit=40))
+    ref = [alone(r) for r in reqs]
+    srv = S.Server(DEV, bucket_lengths=None, streams=2)
+    srv.run(reqs, realtime=False)
+    torch.cuda.synchronize()
+    # classes: gram non-symmetric (2), gram symmetric (2), robust_gram each (1 + 1)
+    assert srv.launches == 4
+    for r, a in zip(reqs, ref):
+        assert r.error is None, r.error
+        assert torch.equal(r.result, a), (r.kind, r.id)
+
+
+def _serving_mix(n, seed, max_batch):
+    return S.synthetic_requests(n, seed=seed, device=DEV, max_batch_size=max_batch,
+                                precision="fp16x2")
+
+
+def test_serving_mix_real_shapes_alone_equals_in_mix_and_oracle():
+    """40 requests of the serving mix at its real shapes (70% SigPDE / 30% RobustSigPDE, pairwise
+    or Gram, lengths 16-512, dims 4/16/64, dyadic orders 0-3, float32, final solves fp16x2), the
+    batch size capped at 12 to keep the oracle cheap. Served with exact lengths, every result
+    equals the direct API call bit for bit; with length buckets (zero-increment padding), within
+    rounding of it (the robust kinds within the float32 root-find's resolution); sampled requests
+    match the float64 oracle."""
+    reqs = _serving_mix(40, 11, 12)
+    kinds = {r.kind for r in reqs}
+    assert {"pairwise", "gram", "robust_pairwise", "robust_gram"} <= kinds
+    assert max(r.dyadic_order for r in reqs) == 3 and max(r.x.shape[1] for r in reqs) > 400
+    ref = [alone(r) for r in reqs]
+    torch.cuda.synchronize()
+    srv = S.Server(DEV, bucket_lengths=None)
+    srv.run(reqs, realtime=False)
+    torch.cuda.synchronize()
+    for r, a in zip(reqs, ref):
+        assert r.error is None, (r.kind, r.error)
+        assert torch.equal(r.result, a), (r.kind, r.id, r.x.shape, r.dyadic_order)
+    exact_launches = srv.launches
+    reqs2 = _serving_mix(40, 11, 12)
+    srv2 = S.Server(DEV)
+    srv2.run(reqs2, realtime=False)
+    torch.cuda.synchronize()
+    # exact lengths rarely share a class in this mix; buckets do
+    assert srv2.launches < len(reqs2) and srv2.launches <= exact_launches
+    for r, a in zip(reqs2, ref):
+        assert r.error is None, (r.kind, r.error)
+        tol = 1e-5 if r.kind in ("pairwise", "gram") else 1e-4
+        torch.testing.assert_close(r.result, a, rtol=tol, atol=1e-5 * float(a.abs().max()) + 1e-6)
+    # sampled requests against the float64 oracle (float64 inputs equal to the float32 paths)
+    checked = 0
+    for r in reqs:
+        cells = S.request_cells(r)
+        if r.kind not in ("pairwise", "gram") or cells > 3e8:
+            continue
+        x = r.x.double().cpu().numpy()
+        y = (r.x if r.y is None else r.y).double().cpu().numpy()
+        want = (O.linear_gram(x, y, r.dyadic_order) if r.kind == "gram"
+                else O.sig_pairwise(x, y, r.dyadic_order))
+        got = r.result.double().cpu().numpy()
+        assert np.linalg.norm(got - want) <= 1e-5 * np.linalg.norm(want), (r.kind, r.id)
+        checked += 1
+    assert checked >= 5

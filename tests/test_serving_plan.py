@@ -106,3 +106,62 @@ def test_synthetic_mix_shapes():
         assert 1 <= r.x.shape[0] <= 256 and 16 <= r.x.shape[1] <= 512
         assert ((max(r.x.shape[1], (r.y if r.y is not None else r.x).shape[1]) - 1)
                 << r.dyadic_order) <= 4096
+
+
+def test_fused_robust_estimate_counts_root_find_self_increments():
+    # the root-find's self-increments exist whatever precision the final solves take
+    for kind in ("robust_pairwise", "normalization"):
+        ref = S.estimate_bytes(kind, "ref", 4096, 4096, 512, 512, 16, 0, 4, 4096)
+        m = 511
+        root_find = 2 * 4096 * (m * m + m * m) * 4
+        assert ref >= root_find
+    fused = S.estimate_bytes("robust_pairwise", "fp16x2", 4096, 4096, 512, 512, 16, 0, 4, 4096)
+    assert fused >= root_find
+
+
+def gram_req(B, L, By=None, d=4, order=0, precision="fp16x2", kind="gram", symmetric=False):
+    x = torch.zeros((B, L, d), dtype=torch.float32)
+    y = None if symmetric else torch.zeros((By or B, L, d), dtype=torch.float32)
+    return S.Request(kind, x, y, dyadic_order=order, precision=precision)
+
+
+def test_fused_gram_requests_coalesce_per_class():
+    srv = S.Server("cpu", max_gram_pairs=1000)
+    a = srv.submit(gram_req(10, 20, 12))
+    b = srv.submit(gram_req(8, 23, 9))          # same class: lengths bucket to 24
+    c = srv.submit(gram_req(5, 20, 7, d=16))    # another dim
+    d = srv.submit(gram_req(6, 20, symmetric=True))   # symmetric: a class of its own
+    e = srv.submit(gram_req(6, 20, 12, precision="ref"))  # reference scheme: alone
+    f = srv.submit(gram_req(30, 20, 30))        # would pass max_gram_pairs with a and b
+    assert S.gram_coalesced(a) and not S.gram_coalesced(e)
+    got = []
+    while True:
+        bt = srv._take_batch()
+        if bt is None:
+            break
+        got.append([r.id for r in bt.reqs])
+    assert [a.id, b.id] in got and [f.id] in got and [c.id] in got and [d.id] in got
+    assert [e.id] in got
+    assert len(got) == 5
+
+
+def test_gram_batch_caps_at_64_requests():
+    srv = S.Server("cpu")
+    for _ in range(70):
+        srv.submit(gram_req(2, 20, 2))
+    b1 = srv._take_batch()
+    assert len(b1.reqs) == 64
+    assert len(srv._take_batch().reqs) == 6
+
+
+def test_cancel_and_timeout_of_queued_requests():
+    srv = S.Server("cpu")
+    a = srv.submit(req("pairwise", 2, 20))
+    b = srv.submit(req("pairwise", 2, 20, order=1))
+    c = srv.submit(req("pairwise", 2, 20, order=2, timeout=0.0))
+    assert srv.cancel(a) is True
+    assert isinstance(a.error, S.CancelledError) and a in srv.completed
+    srv._expire()
+    assert isinstance(c.error, TimeoutError) and c in srv.completed
+    bt = srv._take_batch()
+    assert bt.reqs == [b] and srv._take_batch() is None
