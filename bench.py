@@ -90,6 +90,8 @@ def parse():
                    help="skip the smaller API workloads (pairwise, Gram, robust) beside the line")
     p.add_argument("--serving", type=int, default=300, metavar="N",
                    help="requests of the serving leg (0: skip it)")
+    p.add_argument("--serving-sweep", type=int, default=300, metavar="N",
+                   help="requests of the 50%% / 90%% load sweep (the first N of the same set)")
     p.add_argument("--serving-rate", type=float, default=None,
                    help="Poisson arrival rate of the serving leg, requests per second (default: "
                         "70%% of the saturated rate measured on the same requests)")
@@ -457,39 +459,52 @@ def serving_leg(args, dev):
 
     # one request set (shapes and paths from seed 7) for every run; each rate draws its own
     # Poisson arrival times (seed 8)
+    print(f"serving leg: generating {args.serving} requests", file=sys.stderr, flush=True)
     base = S.synthetic_requests(args.serving, seed=7, device=dev, precision="fp16x2")
     torch.cuda.synchronize()
+    print("serving leg: requests generated", file=sys.stderr, flush=True)
 
-    def serve(rate):
+    def report(done, total, t):
+        print(f"serving leg: {done}/{total} requests served after {t:.0f} s", file=sys.stderr,
+              flush=True)
+
+    def serve(rate, n):
+        sub = base[:n]
         if np.isfinite(rate):
-            arrivals = np.cumsum(np.random.default_rng(8).exponential(1.0 / rate, len(base)))
+            arrivals = np.cumsum(np.random.default_rng(8).exponential(1.0 / rate, len(sub)))
         else:
-            arrivals = np.zeros(len(base))
+            arrivals = np.zeros(len(sub))
         reqs = [S.Request(r.kind, r.x, r.y, r.static_kernel, r.dyadic_order, r.precision,
                           r.normalizer, r.tol, r.maxit, arrival=float(a))
-                for r, a in zip(base, arrivals)]
+                for r, a in zip(sub, arrivals)]
         srv = S.Server(dev, hbm_budget=64 << 30, streams=4)
         t0 = time.perf_counter()
-        srv.run(reqs, realtime=True)
+        srv.run(reqs, realtime=True, progress=report)
         torch.cuda.synchronize()
         out = S.summarize(reqs, time.perf_counter() - t0)
         print(f"serving leg: {len(reqs)} requests at rate {rate:.3g}/s served in "
-              f"{out['wall_s']:.1f} s", file=sys.stderr, flush=True)
+              f"{out['wall_s']:.1f} s ({srv.launches} launches)", file=sys.stderr, flush=True)
         return out, srv.launches
 
-    sat, _ = serve(float("inf"))
+    sat, sat_launches = serve(float("inf"), len(base))
     rate = args.serving_rate if args.serving_rate else 0.7 * sat["requests_per_s"]
-    out, launches = serve(rate)
+    out, launches = serve(rate, len(base))
     sweep = []
+    nsw = min(len(base), max(1, args.serving_sweep))
+    if nsw == len(base):
+        sat_sw = sat
+    else:
+        sat_sw, _ = serve(float("inf"), nsw)
     for frac in (0.5, 0.9):
-        r = frac * sat["requests_per_s"]
-        o, _ = serve(r)
-        sweep.append({"load": frac, "rate_per_s": r, "requests_per_s": o["requests_per_s"],
-                      "pairs_per_s": o["pairs_per_s"], "latency_ms": o["latency_ms"]})
+        r = frac * sat_sw["requests_per_s"]
+        o, _ = serve(r, nsw)
+        sweep.append({"load": frac, "requests": nsw, "rate_per_s": r,
+                      "requests_per_s": o["requests_per_s"], "pairs_per_s": o["pairs_per_s"],
+                      "latency_ms": o["latency_ms"]})
     out.update({"rate_per_s": rate, "launches": launches, "sweep": sweep,
                 "saturated": {"requests_per_s": sat["requests_per_s"],
                               "pairs_per_s": sat["pairs_per_s"], "wall_s": sat["wall_s"],
-                              "latency_ms": sat["latency_ms"]},
+                              "latency_ms": sat["latency_ms"], "launches": sat_launches},
                 "mix": "70% SigPDE / 30% RobustSigPDE, pairwise or Gram 50/50, batch U{1..256}, "
                        "length U{16..512}, dim {4,16,64}, order U{0..3} (capped at a refined side "
                        "of 4096), float32, seed 7, precision fp16x2, 64 GB budget"})

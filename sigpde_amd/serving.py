@@ -542,13 +542,21 @@ class Server:
         self.launches = launches
         del self.completed[done:]
 
-    def run(self, requests: List[Request], realtime: bool = True) -> List[Request]:
+    def run(self, requests: List[Request], realtime: bool = True,
+            progress: Optional[Callable[[int, int, float], None]] = None,
+            progress_every: float = 20.0) -> List[Request]:
         """Serves `requests` (sorted by arrival). realtime: submit each at its arrival time
-        (seconds after start); otherwise all at once."""
+        (seconds after start); otherwise all at once. progress(done, total, seconds) is called
+        about every progress_every seconds while requests remain."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         self.t0 = time.perf_counter()
+        done0 = len(self.completed)
+        next_report = progress_every
         i = 0
         while i < len(reqs) or self.pending():
+            if progress is not None and self.now() >= next_report:
+                progress(len(self.completed) - done0, len(reqs), self.now())
+                next_report = self.now() + progress_every
             now = self.now()
             while i < len(reqs) and (not realtime or reqs[i].arrival <= now):
                 self.submit(reqs[i])
