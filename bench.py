@@ -3,8 +3,10 @@
 Workload (default): Gram of X, Y each 1024 Brownian paths, length 256, dim 64, dyadic order 0,
 linear static kernel (increments N(0, 1/256), seeds 5/6, float32 inputs resident in HBM).
 One step = one full 1024 x 1024 Gram. value = pairs/sec = 1024^2 / step time of the
-mixed-precision path (precision="fp16x2": fp16 hi/lo increment Gram on MFMA, double wavefront).
-Timed beside it on the same inputs:
+mixed-precision path (precision="fp16x2f": fp16 hi/lo increment Gram on MFMA, float32 wavefront;
+--mode selects another). Timed beside it on the same inputs:
+  mixed_fp16x2           precision="fp16x2": the same increments with a double wavefront (the
+                         most accurate mixed mode)
   plain_fp32             precision="fp32": the same fused design in plain FP32 (fp32 increments on
                          fp32 MFMA, the reference's float32 wavefront typing)
   reference_scheme_fp32  precision="ref": the reference's algorithm in float32 (torch materialises
@@ -71,7 +73,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--mode", default="fp16x2", choices=["fp16x2", "bf16x3", "fp32", "ref", "fp64"])
+    p.add_argument("--mode", default="fp16x2f",
+                   choices=["fp16x2", "bf16x3", "fp32", "ref", "fp64", "fp16x2f"])
     p.add_argument("--ref-steps", type=int, default=1,
                    help="timed steps of the reference-scheme FP32 path")
     p.add_argument("--paths", type=int, default=1024)
@@ -149,10 +152,13 @@ def fused_roofline(args, X, Y, ms):
             "fp32": "k_linear_gram_fused<float, 2, 2, 1, false> (fp32 MFMA increments + "
                     "float32-typed wavefront, one launch)",
             "fp64": "k_linear_gram_fused<float, 2, 4, 1, false> (fp64 MFMA increments + double "
-                    "wavefront, one launch)"}[args.mode]
+                    "wavefront, one launch)",
+            "fp16x2f": "k_linear_gram_fused<float, 2, 5, 1, false> (fp16x2 MFMA increments + "
+                       "float32 wavefront, one launch)"}[args.mode]
     traffic = MEASURED_TRAFFIC.get((args.mode, args.paths, args.length, args.dim, args.order))
     alg_bytes = (X.shape[0] * (X.shape[1] - 1) + Y.shape[0] * (Y.shape[1] - 1)) * \
-        ((args.dim + 31) // 32 * 32) * {"fp16x2": 4, "bf16x3": 6, "fp32": 4, "fp64": 8}[args.mode] + \
+        ((args.dim + 31) // 32 * 32) * {"fp16x2": 4, "bf16x3": 6, "fp32": 4, "fp64": 8,
+                                         "fp16x2f": 4}[args.mode] + \
         pairs * X.element_size()
     return {
         "kernel": name,
@@ -170,8 +176,9 @@ def fused_roofline(args, X, Y, ms):
         "increment_gram_mfma_frac": flops_inc / dur_s / 1e12 / (
             {"fp32": MI355X_FP32_MFMA_TFLOPS, "fp64": MI355X_FP64_MFMA_TFLOPS}.get(
                 args.mode, MI355X_FP16_MFMA_TFLOPS)),
-        "note": "bound: the wavefront's FP64 VALU (the cell update is 8 FP64 instructions; peak = "
-                "MI355X FP64 vector 78.6 TFLOP/s, which is also the FP32 rate without packed math); "
+        "note": "bound: the wavefront's VALU (double wavefront: the cell update is 8 FP64 "
+                "instructions; fp16x2f: 7 FP32 instructions; peak = MI355X FP64 vector 78.6 "
+                "TFLOP/s, which is also the FP32 rate without packed math); "
                 "wavefront flops = 10 per refined cell (pde_step as written); the increment Gram "
                 "(2*d flops per increment) runs on MFMA beside it; traffic = measured HBM bytes per "
                 "launch, algorithmic_bytes = operands + output",
@@ -542,12 +549,17 @@ def main():
                   else (K, ms, step_ms))
     Kd, msd, _ = (timed(args, X, Y, "fp64", 1, 1) if args.mode != "fp64" and compare
                   and args.dim <= 64 else (K, ms, step_ms))
+    # the other fp16x2 variant: double wavefront (fp16x2) beside the float32 one (fp16x2f)
+    other16 = "fp16x2f" if args.mode != "fp16x2f" else "fp16x2"
+    f32_wave_ok = args.order <= 2 and ((args.length - 1) << args.order) <= 512
+    K16, ms16, _ = (timed(args, X, Y, other16, args.steps, 1)
+                    if compare and (other16 == "fp16x2" or f32_wave_ok) else (K, ms, step_ms))
 
     roof = (fused_roofline(args, X, Y, ms) if args.mode != "ref"
             else dominant_kernel_roofline(args, X, Y))
     line = {
-        "metric": "Gram pairs/sec (mixed-precision fp16x2 path; plain FP32 beside it) at "
-                  "rel err vs FP64 oracle",
+        "metric": "Gram pairs/sec (mixed-precision path: fp16 hi/lo MFMA increments; plain "
+                  "FP32 beside it) at rel err vs FP64 oracle",
         "value": pairs / (ms / 1e3),
         "unit": "pairs/s",
         "n_gpus": 1,
@@ -558,7 +570,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp16x2": "fp16x2+f64", "bf16x3": "bf16x3+f64", "fp32": "f32",
-                  "ref": "f32", "fp64": "f64"}[args.mode],
+                  "ref": "f32", "fp64": "f64", "fp16x2f": "fp16x2+f32"}[args.mode],
         "data": "synthetic",
         "config": {"workload": f"SigPDE(LinearKernel(), {args.order}).gram(X, Y), X,Y "
                                f"{args.paths} Brownian paths, length {args.length}, dim "
@@ -580,12 +592,19 @@ def main():
                                   "ms_per_step": ms_ref,
                                   "path": "precision='ref': torch increment tiles (point Gram, "
                                           "4-term difference) + generic wavefront"},
+        "mixed_" + other16: {"value": pairs / (ms16 / 1e3), "unit": "pairs/s",
+                             "ms_per_step": ms16,
+                             "path": {"fp16x2": "precision='fp16x2': fp16 hi/lo MFMA increments, "
+                                                "double wavefront",
+                                      "fp16x2f": "precision='fp16x2f': fp16 hi/lo MFMA "
+                                                 "increments, float32 wavefront"}[other16]},
         "roofline": roof,
     }
     if not args.no_cpu:
         acc, rel_others, cpu = cpu_leg(args, X, Y, K, {"plain_fp32": K32,
                                                         "reference_scheme_fp32": Kref,
-                                                        "mixed_bf16x3": Kb})
+                                                        "mixed_bf16x3": Kb,
+                                                        "mixed_" + other16: K16})
         line.update(acc)
         for name, rel in rel_others.items():
             line[name]["rel_err_fp64"] = rel
@@ -607,7 +626,7 @@ def main():
         line["rel_err_fp64_full_reference"] = (
             f"all {pairs} entries vs precision='ref' on float64 inputs (GPU, {t64:.1f} s)")
         for name, Ko in (("plain_fp32", K32), ("reference_scheme_fp32", Kref),
-                         ("mixed_bf16x3", Kb), ("fused_fp64", Kd)):
+                         ("mixed_bf16x3", Kb), ("fused_fp64", Kd), ("mixed_" + other16, K16)):
             if Ko is not K:
                 line[name]["rel_err_fp64_full"] = rel_full(Ko)
         del K64

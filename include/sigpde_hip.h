@@ -114,6 +114,10 @@ int sigpde_robust_norm(int dtype, const void *inc, int64_t B, int64_t m, int64_t
  *     cells stored as float); results in dtype.
  *   precision SIGPDE_PREC_FP64: increments in double on fp64 MFMA, the wavefront in double (the
  *     reference scheme in float64, without the increment tensor); d <= 64; results in dtype.
+ *   precision SIGPDE_PREC_FP16X2F: the fp16x2 increments feeding a float32 wavefront (float
+ *     coefficients and state); dyadic order <= 2 and up to 512 refined rows and columns (other
+ *     grids: SIGPDE_EINVAL), where it stays within 1e-5 of float64; ~14% faster than
+ *     SIGPDE_PREC_FP16X2 at 255 x 255.
  *   X : (Bx, Lx, d), Y : (By, Ly, d) paths of dtype, C-contiguous; 1 <= d <= 128
  *   sx (Bx,), sy (By,) : optional scales of dtype (c *= sx[i] * sy[j]), or NULL
  *   gram: out[i * ldo + j] = K(x_i, y_j); symmetric != 0 requires Y == X and writes the lower
@@ -127,7 +131,8 @@ enum sigpde_precision {
     SIGPDE_PREC_FP16X2 = 1,
     SIGPDE_PREC_FP32 = 2,
     SIGPDE_PREC_BF16X3 = 3,
-    SIGPDE_PREC_FP64 = 4
+    SIGPDE_PREC_FP64 = 4,
+    SIGPDE_PREC_FP16X2F = 5
 };
 
 size_t sigpde_linear_workspace_bytes(int precision, int64_t Bx, int64_t Lx, int64_t By, int64_t Ly,

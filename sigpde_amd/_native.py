@@ -20,7 +20,9 @@ PREC_FP16X2 = 1
 PREC_FP32 = 2
 PREC_BF16X3 = 3
 PREC_FP64 = 4
-PRECISIONS = {"fp16x2": PREC_FP16X2, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "fp64": PREC_FP64}
+PREC_FP16X2F = 5
+PRECISIONS = {"fp16x2": PREC_FP16X2, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "fp64": PREC_FP64,
+              "fp16x2f": PREC_FP16X2F}
 
 _lib = None
 _lock = threading.Lock()

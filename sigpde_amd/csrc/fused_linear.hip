@@ -484,6 +484,9 @@ static hipError_t dispatch_prec(int precision, bool pairwise, const void *X, con
   if (precision == SIGPDE_PREC_FP64)
     return dispatch_kb<T, kPrecFp64>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
                                      symmetric, out, ldo, w, batch, st);
+  if (precision == SIGPDE_PREC_FP16X2F)
+    return dispatch_kb<T, kPrecFp16x2F>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
+                                        symmetric, out, ldo, w, batch, st);
   return dispatch_kb<T, kPrecFp16x2>(pairwise, X, Y, Bx, By, Lx, Ly, d, kscale, order, sx, sy,
                                      symmetric, out, ldo, w, batch, st);
 }
@@ -506,8 +509,17 @@ static int64_t fused_max_columns(int precision) {
 
 static bool known_precision(int precision) {
   return precision == SIGPDE_PREC_FP16X2 || precision == SIGPDE_PREC_FP32 ||
-         precision == SIGPDE_PREC_BF16X3 || precision == SIGPDE_PREC_FP64;
+         precision == SIGPDE_PREC_BF16X3 || precision == SIGPDE_PREC_FP64 ||
+         precision == SIGPDE_PREC_FP16X2F;
 }
+
+// The float32 wavefront (SIGPDE_PREC_FP16X2F) holds 1e-5 against float64 up to 512 refined rows
+// and columns at dyadic order <= 2 (measured on the CPU: 4e-6 at 255 x 255 cells, 5.5e-6 at
+// 396 x 396 at dyadic order 2, 8.6e-6 at 511 x 511; at order 3 the increments c/64 leave c^2/12
+// near float resolution: 4.5e-5 at 1016 x 1016, 1e-4 on single entries at 256 x 256, measured on
+// the GPU); other grids are refused.
+constexpr int64_t kMaxRefinedF32Wave = 512;
+constexpr int kMaxOrderF32Wave = 2;
 
 static int check_fused(int dtype, int precision, const void *X, const void *Y, int64_t Bx,
                        int64_t By, int64_t Lx, int64_t Ly, int64_t d, int order, void *out,
@@ -525,6 +537,17 @@ static int check_fused(int dtype, int precision, const void *X, const void *Y, i
   if (precision == SIGPDE_PREC_FP64 && d > kMaxDimFp64)
     return set_error(SIGPDE_EINVAL, "linear: precision fp64 takes dim <= %d", (int)kMaxDimFp64);
   if (order < 0 || order > 12) return set_error(SIGPDE_EINVAL, "linear: dyadic order must be in [0, 12]");
+  if (precision == SIGPDE_PREC_FP16X2F && order > kMaxOrderF32Wave)
+    return set_error(SIGPDE_EINVAL,
+                     "linear: precision fp16x2f (float32 wavefront) takes dyadic order <= %d, got "
+                     "%d; use fp16x2 (double wavefront)", kMaxOrderF32Wave, order);
+  if (precision == SIGPDE_PREC_FP16X2F &&
+      (((Lx - 1) << order) > kMaxRefinedF32Wave || ((Ly - 1) << order) > kMaxRefinedF32Wave))
+    return set_error(SIGPDE_EINVAL,
+                     "linear: precision fp16x2f (float32 wavefront) takes at most %lld refined "
+                     "rows and columns, got %lld x %lld; use fp16x2 (double wavefront)",
+                     (long long)kMaxRefinedF32Wave, (long long)((Lx - 1) << order),
+                     (long long)((Ly - 1) << order));
   if (((Lx - 1) << order) > kMaxRefinedRows)
     return set_error(SIGPDE_EINVAL, "linear: refined rows (Lx-1) << order = %lld exceed %lld",
                      (long long)((Lx - 1) << order), (long long)kMaxRefinedRows);

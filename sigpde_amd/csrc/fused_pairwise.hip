@@ -84,6 +84,8 @@ hipError_t launch_linear_pairwise_fused(const GramFusedArgs<T> &a, const double 
 }
 
 #define SIGPDE_PAIRWISE_INST(T, KB)                                                               \
+  template hipError_t launch_linear_pairwise_fused<T, KB, kPrecFp16x2F>(                         \
+      const GramFusedArgs<T> &, const double *, const double *, hipStream_t);                      \
   template hipError_t launch_linear_pairwise_fused<T, KB, kPrecFp16x2>(                          \
       const GramFusedArgs<T> &, const double *, const double *, hipStream_t);          \
   template hipError_t launch_linear_pairwise_fused<T, KB, kPrecFp32>(                            \

@@ -25,6 +25,7 @@ constexpr int kPrecFp16x2 = 1;  // per-path-scaled fp16 hi + lo, 3 f16 MFMAs per
 constexpr int kPrecFp32 = 2;    // fp32 increments, v_mfma_f32_16x16x4f32
 constexpr int kPrecBf16x3 = 3;  // bf16 hi + mid + lo, 6 bf16 MFMAs per 32-wide K block
 constexpr int kPrecFp64 = 4;    // double increments, v_mfma_f64_16x16x4f64
+constexpr int kPrecFp16x2F = 5; // the fp16x2 increments feeding a float32 wavefront (Arith::MixedF)
 
 // ------------------------------------------------------------------ MFMA producer (G = 16)
 // Lane group g (16 lanes) solves pairs (x path xa[g], y path ybase[ys(g)] + sp * ystep) of the
@@ -58,7 +59,7 @@ struct MfmaProducer {
   static_assert((NX == 4 && NY == 1) || (NX == 4 && NY == 4), "operand sets");
   __device__ static constexpr int xs(int g) { return g; }
   __device__ static constexpr int ys(int g) { return NY == 4 ? g : 0; }
-  static constexpr bool kF16 = PREC == kPrecFp16x2;
+  static constexpr bool kF16 = PREC == kPrecFp16x2 || PREC == kPrecFp16x2F;
   static constexpr bool kB3 = PREC == kPrecBf16x3;
   static constexpr bool kF64 = PREC == kPrecFp64;
   static constexpr bool kSplit = kF16 || kB3;     // 16-bit parts, pair scale applied by Mixed
@@ -293,7 +294,9 @@ struct GramFusedArgs {
 template <int PREC>
 struct FusedArith {
   static constexpr Arith A =
-      PREC == kPrecFp32 ? Arith::Ref32 : (PREC == kPrecFp64 ? Arith::Ref64 : Arith::Mixed);
+      PREC == kPrecFp32 ? Arith::Ref32
+                        : (PREC == kPrecFp64 ? Arith::Ref64
+                                             : (PREC == kPrecFp16x2F ? Arith::MixedF : Arith::Mixed));
 };
 
 // Where a fused launch keeps its band-boundary lines: in LDS (r16_lds_bytes) unless the launch

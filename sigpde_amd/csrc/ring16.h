@@ -186,10 +186,11 @@ struct R16In {
 
 template <Arith A, int R>
 struct R16State {
-  double k[R];    // the lane's rows at its current column
-  double kPrev;   // the row above the lane's first row, at the previous column
-  double h, q;    // Mixed: coefficients of the lane's current pair
-  double hN, qN;  // Mixed: coefficients taken at the lane's next reset
+  using VT = typename ArithTraits<A>::VT;
+  VT k[R];    // the lane's rows at its current column
+  VT kPrev;   // the row above the lane's first row, at the previous column
+  VT h, q;    // Mixed, MixedF: coefficients of the lane's current pair
+  VT hN, qN;  // Mixed, MixedF: coefficients taken at the lane's next reset
 };
 
 template <bool R, bool O, bool C>
@@ -242,6 +243,36 @@ __device__ __forceinline__ void r16_group(const char *rb, const R16Period<typena
   else
     r16_load<A>(rb, 16 * (1 - PAR), pp.bvNext, nxt);
   BT vals[4];
+  if constexpr (A == Arith::MixedF) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int sidx = 4 * I + k;
+      float a, b;
+      coefs_f(cur.c[k], st.h, st.q, a, b);
+      const float up = shift_down_ff<16>(st.k[R - 1], cur.bv[k]);
+      bool z = false;
+      if constexpr (F::kReset) z = pp.rsi == sidx;
+      float above = up, aboveOld = st.kPrev;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float old = st.k[r];
+        float kn = cell_f(above, old, aboveOld, a, b);
+        if constexpr (F::kReset) kn = z ? 1.0f : kn;
+        st.k[r] = kn;
+        aboveOld = old;
+        above = kn;
+      }
+      st.kPrev = up;
+      if constexpr (F::kReset) {
+        st.h = z ? st.hN : st.h;
+        st.q = z ? st.qN : st.q;
+      }
+      if constexpr (F::kCapture) {
+        if (sidx == pp.sCap) result = st.k[R - 1];
+      }
+      vals[k] = st.k[R - 1];
+    }
+  } else {
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int sidx = 4 * I + k;
@@ -275,6 +306,7 @@ __device__ __forceinline__ void r16_group(const char *rb, const R16Period<typena
       if (sidx == pp.sCap) result = st.k[R - 1];
     }
     vals[k] = (BT)st.k[R - 1];
+  }
   }
   if (rho == 15) {
     BT *w = pp.w + 4 * I;
@@ -373,7 +405,7 @@ __device__ void ring16_solve(Producer &P, const PairGeom<16> &g, double s, void 
     const bool cap = sp < g.stream && (tM >> 4) == per;
     pp.sCap = cap ? tM & 15 : -1;
     pp.rsi = reset ? rho : -1;
-    if constexpr (A == Arith::Mixed) {
+    if constexpr (A == Arith::Mixed || A == Arith::MixedF) {
       if (reset && band == 0) {
         const double m = P.lane_scale();
         st.hN = 0.5 * m;

@@ -77,6 +77,11 @@ template <int G>
 __device__ __forceinline__ double shift_down_f(double v, float old) {
   return (double)__int_as_float(dpp_down1<G>(__float_as_int((float)v), __float_as_int(old)));
 }
+// The float32 wavefront's shift (Arith::MixedF): a float shifted as a float.
+template <int G>
+__device__ __forceinline__ float shift_down_ff(float v, float old) {
+  return __int_as_float(dpp_down1<G>(__float_as_int(v), __float_as_int(old)));
+}
 __device__ __forceinline__ double readlane_d(double v, int l) {
   int2 a = *reinterpret_cast<int2 *>(&v);
   int2 r;
@@ -94,20 +99,66 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 // Mixed: ring holds the raw float increment product p of the split-precision increment Gram; the
 //        pair's scale m (c = m * p) is folded into per-lane double coefficients h = m/2 and
 //        q = m^2/12; double arithmetic and state.
-enum class Arith { Ref32, Ref64, Scaled32, Scaled64, Mixed };
+// MixedF: as Mixed with a float32 wavefront (float coefficients, arithmetic and state; 16-lane
+//        groups only, ring16.h). The cell is written with the increment terms apart from the
+//        ones, k = k01 + (k01 a + ((k10 - k00) + k10 a + k00 b)), a = c/2 + c^2/12, b = c^2/12:
+//        c/2 and c^2/12 keep float precision instead of being rounded against 1 (which loses
+//        c^2/12 entirely at dyadic order 2 and more), and k10 - k00 of neighbouring cells is
+//        nearly exact.
+enum class Arith { Ref32, Ref64, Scaled32, Scaled64, Mixed, MixedF };
 
+// RT: ring element. VT: the wavefront's cell values and coefficients.
 template <Arith A>
 struct ArithTraits;
 template <>
-struct ArithTraits<Arith::Ref32> { using RT = float; static constexpr bool kRound = true; };
+struct ArithTraits<Arith::Ref32> {
+  using RT = float;
+  using VT = double;
+  static constexpr bool kRound = true;
+};
 template <>
-struct ArithTraits<Arith::Ref64> { using RT = double; static constexpr bool kRound = false; };
+struct ArithTraits<Arith::Ref64> {
+  using RT = double;
+  using VT = double;
+  static constexpr bool kRound = false;
+};
 template <>
-struct ArithTraits<Arith::Scaled32> { using RT = float; static constexpr bool kRound = true; };
+struct ArithTraits<Arith::Scaled32> {
+  using RT = float;
+  using VT = double;
+  static constexpr bool kRound = true;
+};
 template <>
-struct ArithTraits<Arith::Scaled64> { using RT = double; static constexpr bool kRound = false; };
+struct ArithTraits<Arith::Scaled64> {
+  using RT = double;
+  using VT = double;
+  static constexpr bool kRound = false;
+};
 template <>
-struct ArithTraits<Arith::Mixed> { using RT = float; static constexpr bool kRound = false; };
+struct ArithTraits<Arith::Mixed> {
+  using RT = float;
+  using VT = double;
+  static constexpr bool kRound = false;
+};
+template <>
+struct ArithTraits<Arith::MixedF> {
+  using RT = float;
+  using VT = float;
+  static constexpr bool kRound = false;
+};
+
+// MixedF: coefficients of the float32 wavefront from the raw product p (c = m p, h = m/2,
+// q = m^2/12): b = q p^2, a = h p + b.
+__device__ __forceinline__ void coefs_f(float p, float h, float q, float &a, float &b) {
+  b = q * (p * p);
+  a = __builtin_fmaf(h, p, b);
+}
+// MixedF cell: k = k01 + (k01 a + w), w = (k10 - k00) + k10 a + k00 b (w does not wait for the
+// value shifted in from the row above).
+__device__ __forceinline__ float cell_f(float k01, float k10, float k00, float a, float b) {
+  const float w = __builtin_fmaf(k00, b, __builtin_fmaf(k10, a, k10 - k00));
+  return __builtin_fmaf(k01, a, w) + k01;
+}
 
 // A = (1 + c/2) + c^2/12 and B = 1 - c^2/12 as pde_step writes them, with the products contracted
 // into FMAs as NVVM contracts the reference's expression (fma(c, 0.5, 1) equals 1 + 0.5 c: the

@@ -669,7 +669,7 @@ def main(argv=None):
     p.add_argument("--budget-gb", type=float, default=64.0)
     p.add_argument("--streams", type=int, default=4)
     p.add_argument("--precision", default="ref",
-                   choices=["ref", "fp16x2", "bf16x3", "fp32", "fp64"])
+                   choices=["ref", "fp16x2", "bf16x3", "fp32", "fp64", "fp16x2f"])
     p.add_argument("--graphs", action="store_true")
     p.add_argument("--max-length", type=int, default=512)
     args = p.parse_args(argv)

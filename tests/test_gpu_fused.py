@@ -9,7 +9,9 @@ inputs upcast:
     typing): float32 storage of every cell bounds it; bar 1e-4 Frobenius / 1e-3 per entry;
   precision="fp64" (double increments on fp64 MFMA, double wavefront, dim <= 64): the float32
     rounding of the result bounds it on float32 inputs (bar 2e-7); on float64 inputs it is the
-    float64 reference scheme (bar 1e-12).
+    float64 reference scheme (bar 1e-12);
+  precision="fp16x2f" (the fp16x2 increments, float32 wavefront, refined grids up to 512 x 512):
+    the headline bar, 1e-5 Frobenius (measured 2e-6..9e-6 on the CPU model of its arithmetic).
 """
 import numpy as np
 import pytest
@@ -22,7 +24,7 @@ from sigpde_amd import _native
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda"
-BARS = {"fp16x2": 1e-5, "bf16x3": 1e-5, "fp32": 1e-4, "fp64": 2e-7}
+BARS = {"fp16x2": 1e-5, "bf16x3": 1e-5, "fp32": 1e-4, "fp64": 2e-7, "fp16x2f": 1e-5}
 PRECS = list(BARS)
 
 
@@ -52,6 +54,10 @@ def test_fused_gram_vs_f64(L, d, order, var, prec):
     k = sig.SigPDE(sig.kernels.LinearKernel(), order, precision=prec)
     if prec == "fp64" and d > 64:
         with pytest.raises(_native.NativeError, match="dim <= 64"):
+            k.gram(cuda(x), cuda(y))
+        return
+    if prec == "fp16x2f" and order > 2:
+        with pytest.raises(_native.NativeError, match="dyadic order <= 2"):
             k.gram(cuda(x), cuda(y))
         return
     got = k.gram(cuda(x), cuda(y)).cpu().numpy().astype(np.float64)
@@ -143,11 +149,15 @@ def test_fused_long_rows_of_columns(prec, L, order):
     (measured ~2e-7 at 2000 increments: their products of each magnitude are accumulated apart, so
     no rounding bias builds up along a path); fp32 gets 3e-4 (its float32 wavefront leaves ~2e-4
     absolute error on entries near 0.1 of paths this long); fp64 keeps 2e-7."""
-    bar = {"fp16x2": 1e-5, "bf16x3": 1e-5, "fp32": 3e-4, "fp64": 2e-7}[prec]
     x = brownian_np(5, L, 4, 1 / L, 21, np.float32)
     y = brownian_np(3, L, 4, 1 / L, 22, np.float32)
-    x64, y64 = x.astype(np.float64), y.astype(np.float64)
     k = sig.SigPDE(sig.kernels.LinearKernel(), order, precision=prec)
+    if prec == "fp16x2f":  # past its 512 refined columns / order 2
+        with pytest.raises(_native.NativeError, match="fp16x2f"):
+            k.gram(cuda(x), cuda(y))
+        return
+    bar = {"fp16x2": 1e-5, "bf16x3": 1e-5, "fp32": 3e-4, "fp64": 2e-7}[prec]
+    x64, y64 = x.astype(np.float64), y.astype(np.float64)
     ref = O.linear_gram(x64, y64, order)
     check(k.gram(cuda(x), cuda(y)).cpu().numpy(), ref, prec, bar)
     check(k.pairwise(cuda(x[:3]), cuda(y)).cpu().numpy(), np.diag(ref[:3, :3]).copy(), prec, bar)
@@ -162,7 +172,7 @@ def test_fused_rejects_nonlinear_kernel(prec):
         sig.SigPDE(sig.kernels.RBFKernel(), 0, precision=prec)
 
 
-@pytest.mark.parametrize("prec", ["fp16x2", "bf16x3", "fp64"])
+@pytest.mark.parametrize("prec", ["fp16x2", "bf16x3", "fp64", "fp16x2f"])
 def test_fused_full_1024x1024_gram_vs_gpu_float64(prec):
     """Every entry of the 1024 x 1024 Gram (length 256, dim 64, order 0) against the float64 GPU
     path (precision="ref" on the float64 inputs: the reference scheme in float64, pinned to the CPU
@@ -178,3 +188,17 @@ def test_fused_full_1024x1024_gram_vs_gpu_float64(prec):
     assert rel_fro(K64[:16, :16].cpu().numpy(), ref_corner) < 1e-12
     rel = float(torch.linalg.norm(K.double() - K64) / torch.linalg.norm(K64))
     assert rel < BARS[prec], rel
+
+
+def test_fp16x2f_refuses_grids_past_512_refined_cells_a_side():
+    x = brownian_np(2, 130, 4, 1 / 130, 3, np.float32)   # 129 << 2 = 516 refined rows
+    k = sig.SigPDE(sig.kernels.LinearKernel(), 2, precision="fp16x2f")
+    with pytest.raises(_native.NativeError, match="at most 512 refined"):
+        k.gram(cuda(x), cuda(x))
+    with pytest.raises(_native.NativeError, match="dyadic order <= 2"):
+        sig.SigPDE(sig.kernels.LinearKernel(), 3, precision="fp16x2f").gram(cuda(x[:, :33]))
+    # the same paths at order 1 (258 refined rows) run
+    k1 = sig.SigPDE(sig.kernels.LinearKernel(), 1, precision="fp16x2f")
+    got = k1.gram(cuda(x)).cpu().numpy().astype(np.float64)
+    ref = O.linear_gram(x.astype(np.float64), x.astype(np.float64), 1)
+    check(got, ref, "fp16x2f")

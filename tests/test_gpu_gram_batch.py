@@ -19,7 +19,8 @@ def paths(B, L, d, seed, dtype=torch.float32):
 
 
 @pytest.mark.parametrize("precision,dtype", [("fp16x2", torch.float32), ("bf16x3", torch.float32),
-                                             ("fp32", torch.float32), ("fp64", torch.float64)])
+                                             ("fp32", torch.float32), ("fp64", torch.float64),
+                                             ("fp16x2f", torch.float32)])
 @pytest.mark.parametrize("order", [0, 2])
 def test_batch_equals_separate_grams(precision, dtype, order):
     L, d = 40, 16
