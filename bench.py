@@ -62,10 +62,12 @@ MI355X_FP16_MFMA_TFLOPS = 2500.0
 MI355X_FP32_MFMA_TFLOPS = 157.3
 MI355X_FP64_MFMA_TFLOPS = 78.6
 FLOPS_PER_CELL = 10  # pde_step as written
-# HBM bytes per launch of the fused fp16x2 Gram at the default workload (1024 x 1024 paths, length
-# 256, dim 64, order 0), from rocprofv3 FETCH_SIZE + WRITE_SIZE passes scaled by a 256 MiB copy
-# (profiles/gram_fp16x2_1024x1024_L256_d64_o0_traffic.md); other workloads report null.
-MEASURED_TRAFFIC = {("fp16x2", 1024, 256, 64, 0): 7.718e9}
+# HBM bytes per launch of the fused fp16x2 and fp16x2f Grams at the default workload (1024 x 1024
+# paths, length 256, dim 64, order 0), from rocprofv3 FETCH_SIZE + WRITE_SIZE passes scaled by a
+# 256 MiB copy (profiles/gram_fp16x2*_1024x1024_L256_d64_o0_traffic.md); other workloads report
+# null.
+MEASURED_TRAFFIC = {("fp16x2", 1024, 256, 64, 0): 7.718e9,
+                    ("fp16x2f", 1024, 256, 64, 0): 10.815e9}
 
 
 def parse():
@@ -177,8 +179,9 @@ def fused_roofline(args, X, Y, ms):
             {"fp32": MI355X_FP32_MFMA_TFLOPS, "fp64": MI355X_FP64_MFMA_TFLOPS}.get(
                 args.mode, MI355X_FP16_MFMA_TFLOPS)),
         "note": "bound: the wavefront's VALU (double wavefront: the cell update is 8 FP64 "
-                "instructions; fp16x2f: 7 FP32 instructions; peak = MI355X FP64 vector 78.6 "
-                "TFLOP/s, which is also the FP32 rate without packed math); "
+                "instructions; fp16x2f: 8 FP32 instructions; peak = MI355X FP64 vector 78.6 "
+                "TFLOP/s, which is also the FP32 rate without packed math: the FP32 vector peak "
+                "of 157.3 TFLOP/s needs packed FP32 instructions, which this build turns off); "
                 "wavefront flops = 10 per refined cell (pde_step as written); the increment Gram "
                 "(2*d flops per increment) runs on MFMA beside it; traffic = measured HBM bytes per "
                 "launch, algorithmic_bytes = operands + output",
@@ -539,27 +542,39 @@ def main():
 
     K, ms, step_ms = timed(args, X, Y, args.mode, args.steps, args.warmup)
     pairs = X.shape[0] * Y.shape[0]
-    # comparison paths on the same workload
+    # comparison paths on the same workload (each leg only when it differs from --mode; none
+    # with --no-compare)
     compare = not args.no_compare
-    K32, ms32, _ = (timed(args, X, Y, "fp32", args.steps, 1) if args.mode != "fp32" and compare
-                    else (K, ms, step_ms))
-    Kref, ms_ref, _ = (timed(args, X, Y, "ref", args.ref_steps, 1) if args.mode != "ref" and compare
-                       else (K, ms, step_ms))
-    Kb, msb, _ = (timed(args, X, Y, "bf16x3", args.steps, 1) if args.mode != "bf16x3" and compare
-                  else (K, ms, step_ms))
-    Kd, msd, _ = (timed(args, X, Y, "fp64", 1, 1) if args.mode != "fp64" and compare
-                  and args.dim <= 64 else (K, ms, step_ms))
-    # the other fp16x2 variant: double wavefront (fp16x2) beside the float32 one (fp16x2f)
     other16 = "fp16x2f" if args.mode != "fp16x2f" else "fp16x2"
     f32_wave_ok = args.order <= 2 and ((args.length - 1) << args.order) <= 512
-    K16, ms16, _ = (timed(args, X, Y, other16, args.steps, 1)
-                    if compare and (other16 == "fp16x2" or f32_wave_ok) else (K, ms, step_ms))
+    legs = [  # (line key, precision, timed steps, runs at this shape, path description)
+        ("plain_fp32", "fp32", args.steps, True,
+         "precision='fp32': fused, fp32 MFMA increments, float32 wavefront"),
+        ("reference_scheme_fp32", "ref", args.ref_steps, True,
+         "precision='ref': torch increment tiles (point Gram, 4-term difference) + generic "
+         "wavefront"),
+        ("mixed_bf16x3", "bf16x3", args.steps, True,
+         "precision='bf16x3': three-part bf16 increments, six bf16 MFMA products, double "
+         "wavefront"),
+        ("fused_fp64", "fp64", 1, args.dim <= 64,
+         "precision='fp64': double increments on fp64 MFMA, double wavefront (the reference "
+         "scheme in float64, fused)"),
+        ("mixed_" + other16, other16, args.steps, other16 == "fp16x2" or f32_wave_ok,
+         {"fp16x2": "precision='fp16x2': fp16 hi/lo MFMA increments, double wavefront",
+          "fp16x2f": "precision='fp16x2f': fp16 hi/lo MFMA increments, float32 wavefront"}[
+             other16]),
+    ]
+    others = {}  # line key -> (K, ms, path)
+    for key, prec, steps, ok, path in legs:
+        if compare and ok and prec != args.mode:
+            Ko, mso, _ = timed(args, X, Y, prec, steps, 1)
+            others[key] = (Ko, mso, path)
 
     roof = (fused_roofline(args, X, Y, ms) if args.mode != "ref"
             else dominant_kernel_roofline(args, X, Y))
     line = {
-        "metric": "Gram pairs/sec (mixed-precision path: fp16 hi/lo MFMA increments; plain "
-                  "FP32 beside it) at rel err vs FP64 oracle",
+        "metric": "Gram pairs/sec vs plain FP32 at \u22641e-5 rel err (FP64 oracle); p50/p99 "
+                  "latency",
         "value": pairs / (ms / 1e3),
         "unit": "pairs/s",
         "n_gpus": 1,
@@ -579,32 +594,16 @@ def main():
                    "dim": args.dim, "dyadic_order": args.order},
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
                        "p99": float(np.percentile(step_ms, 99))},
-        "plain_fp32": {"value": pairs / (ms32 / 1e3), "unit": "pairs/s", "ms_per_step": ms32,
-                       "path": "precision='fp32': fused, fp32 MFMA increments, float32 wavefront"},
-        "speedup_vs_plain_fp32": ms32 / ms,
-        "mixed_bf16x3": {"value": pairs / (msb / 1e3), "unit": "pairs/s", "ms_per_step": msb,
-                         "path": "precision='bf16x3': three-part bf16 increments, six bf16 MFMA "
-                                 "products, double wavefront"},
-        "fused_fp64": {"value": pairs / (msd / 1e3), "unit": "pairs/s", "ms_per_step": msd,
-                       "path": "precision='fp64': double increments on fp64 MFMA, double "
-                               "wavefront (the reference scheme in float64, fused)"},
-        "reference_scheme_fp32": {"value": pairs / (ms_ref / 1e3), "unit": "pairs/s",
-                                  "ms_per_step": ms_ref,
-                                  "path": "precision='ref': torch increment tiles (point Gram, "
-                                          "4-term difference) + generic wavefront"},
-        "mixed_" + other16: {"value": pairs / (ms16 / 1e3), "unit": "pairs/s",
-                             "ms_per_step": ms16,
-                             "path": {"fp16x2": "precision='fp16x2': fp16 hi/lo MFMA increments, "
-                                                "double wavefront",
-                                      "fp16x2f": "precision='fp16x2f': fp16 hi/lo MFMA "
-                                                 "increments, float32 wavefront"}[other16]},
         "roofline": roof,
     }
+    for key, (Ko, mso, path) in others.items():
+        line[key] = {"value": pairs / (mso / 1e3), "unit": "pairs/s", "ms_per_step": mso,
+                     "path": path}
+    if "plain_fp32" in others:
+        line["speedup_vs_plain_fp32"] = others["plain_fp32"][1] / ms
     if not args.no_cpu:
-        acc, rel_others, cpu = cpu_leg(args, X, Y, K, {"plain_fp32": K32,
-                                                        "reference_scheme_fp32": Kref,
-                                                        "mixed_bf16x3": Kb,
-                                                        "mixed_" + other16: K16})
+        acc, rel_others, cpu = cpu_leg(args, X, Y, K, {key: o[0] for key, o in others.items()
+                                                        if key != "fused_fp64"})
         line.update(acc)
         for name, rel in rel_others.items():
             line[name]["rel_err_fp64"] = rel
@@ -625,10 +624,8 @@ def main():
         line["rel_err_fp64_full"] = rel_full(K)
         line["rel_err_fp64_full_reference"] = (
             f"all {pairs} entries vs precision='ref' on float64 inputs (GPU, {t64:.1f} s)")
-        for name, Ko in (("plain_fp32", K32), ("reference_scheme_fp32", Kref),
-                         ("mixed_bf16x3", Kb), ("fused_fp64", Kd), ("mixed_" + other16, K16)):
-            if Ko is not K:
-                line[name]["rel_err_fp64_full"] = rel_full(Ko)
+        for name, (Ko, _, _) in others.items():
+            line[name]["rel_err_fp64_full"] = rel_full(Ko)
         del K64
     if not args.no_workloads:
         line["workloads"] = workloads_leg(args, dev)
