@@ -126,6 +126,14 @@ def timed(args, X, Y, mode, steps, warmup):
     return K, total / steps * 1e3, step_ms
 
 
+def _point_ne(d):
+    """K elements per lane of the fused static-kernel Gram (float32): dpad = 4 NE >= d."""
+    ne = 2
+    while 4 * ne < d:
+        ne *= 2
+    return ne
+
+
 def fused_roofline(args, X, Y, ms):
     """The fused kernel is one launch per step (plus a small split prepass); its time is the step
     time measured with HIP events on torch's current stream, the stream it is launched on."""
@@ -146,7 +154,9 @@ def fused_roofline(args, X, Y, ms):
     pairs = X.shape[0] * Y.shape[0]
     cells = pairs * m * n
     flops_wave = cells * FLOPS_PER_CELL
-    flops_inc = pairs * 2 * args.dim * (X.shape[1] - 1) * (Y.shape[1] - 1)
+    # MFMA work beside the wavefront: the increment Gram, or for "ref" the point Gram
+    flops_inc = pairs * 2 * args.dim * (X.shape[1] - (args.mode != "ref")) * \
+        (Y.shape[1] - (args.mode != "ref"))
     name = {"fp16x2": "k_linear_gram_fused<float, 2, 1, 1, false> (fp16x2 MFMA increments + "
                       "double wavefront, one launch)",
             "bf16x3": "k_linear_gram_fused<float, 2, 3, 1, false> (bf16x3 MFMA increments + "
@@ -156,12 +166,19 @@ def fused_roofline(args, X, Y, ms):
             "fp64": "k_linear_gram_fused<float, 2, 4, 1, false> (fp64 MFMA increments + double "
                     "wavefront, one launch)",
             "fp16x2f": "k_linear_gram_fused<float, 2, 5, 1, false> (fp16x2 MFMA increments + "
-                       "float32 wavefront, one launch)"}[args.mode]
+                       "float32 wavefront, one launch)",
+            "ref": f"k_point_gram<float, 0, {_point_ne(args.dim)}, {min(1 << args.order, 8)}> "
+                   "(fp32 MFMA point Gram, second difference and float32-typed wavefront, one "
+                   "launch)"}[args.mode]
     traffic = MEASURED_TRAFFIC.get((args.mode, args.paths, args.length, args.dim, args.order))
-    alg_bytes = (X.shape[0] * (X.shape[1] - 1) + Y.shape[0] * (Y.shape[1] - 1)) * \
-        ((args.dim + 31) // 32 * 32) * {"fp16x2": 4, "bf16x3": 6, "fp32": 4, "fp64": 8,
-                                         "fp16x2f": 4}[args.mode] + \
-        pairs * X.element_size()
+    if args.mode == "ref":  # padded float points of X and Y, and the output
+        alg_bytes = (X.shape[0] * X.shape[1] + Y.shape[0] * Y.shape[1]) * \
+            4 * _point_ne(args.dim) * 4 + pairs * X.element_size()
+    else:  # the split increments of X and Y, and the output
+        alg_bytes = (X.shape[0] * (X.shape[1] - 1) + Y.shape[0] * (Y.shape[1] - 1)) * \
+            ((args.dim + 31) // 32 * 32) * {"fp16x2": 4, "bf16x3": 6, "fp32": 4, "fp64": 8,
+                                             "fp16x2f": 4}[args.mode] + \
+            pairs * X.element_size()
     return {
         "kernel": name,
         "bound": "valu",
@@ -176,7 +193,8 @@ def fused_roofline(args, X, Y, ms):
         "cells_per_s": cells / dur_s,
         "increment_gram_tflops": flops_inc / dur_s / 1e12,
         "increment_gram_mfma_frac": flops_inc / dur_s / 1e12 / (
-            {"fp32": MI355X_FP32_MFMA_TFLOPS, "fp64": MI355X_FP64_MFMA_TFLOPS}.get(
+            {"fp32": MI355X_FP32_MFMA_TFLOPS, "ref": MI355X_FP32_MFMA_TFLOPS,
+             "fp64": MI355X_FP64_MFMA_TFLOPS}.get(
                 args.mode, MI355X_FP16_MFMA_TFLOPS)),
         "note": "bound: the wavefront's VALU (double wavefront: the cell update is 8 FP64 "
                 "instructions; fp16x2f: 8 FP32 instructions; peak = MI355X FP64 vector 78.6 "
@@ -185,48 +203,6 @@ def fused_roofline(args, X, Y, ms):
                 "wavefront flops = 10 per refined cell (pde_step as written); the increment Gram "
                 "(2*d flops per increment) runs on MFMA beside it; traffic = measured HBM bytes per "
                 "launch, algorithmic_bytes = operands + output",
-    }
-
-
-def dominant_kernel_roofline(args, X, Y):
-    """Times the wavefront kernel alone on one increment tile with HIP events on torch's current
-    stream (the stream the library launches on)."""
-    tile = int(np.ceil(np.sqrt(args.max_batch)))
-    tx = min(tile, X.shape[0])
-    ty = min(tile, Y.shape[0])
-    from sigpde_amd.torch.kernel import gram_inner_product
-    inc = gram_inner_product(X[:tx], Y[:ty], sig.kernels.LinearKernel(), args.order).contiguous()
-    out = torch.zeros((tx, ty), device=X.device, dtype=X.dtype)
-    _native.pde_gram(inc, args.order, out)
-    torch.cuda.synchronize()
-    reps = 3
-    st = torch.cuda.current_stream()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(st)
-    for _ in range(reps):
-        _native.pde_gram(inc, args.order, out)
-    e1.record(st)
-    torch.cuda.synchronize()
-    dur_s = e0.elapsed_time(e1) / 1e3 / reps
-    m = (X.shape[1] - 1) << args.order
-    n = (Y.shape[1] - 1) << args.order
-    pairs = tx * ty
-    # algorithmic HBM bytes per launch: the increment tile read once + the output written
-    bytes_ = pairs * ((X.shape[1] - 1) * (Y.shape[1] - 1) + 1) * X.element_size()
-    flops = pairs * m * n * FLOPS_PER_CELL
-    return {
-        "kernel": "k_pde_gram (wavefront from HBM increments)",
-        "bound": "hbm",
-        "achieved": bytes_ / dur_s / 1e9,
-        "peak": MI355X_HBM_GBS,
-        "unit": "GB/s",
-        "frac": bytes_ / dur_s / 1e9 / MI355X_HBM_GBS,
-        "traffic": None,
-        "launch_ms": dur_s * 1e3,
-        "pairs_per_launch": pairs,
-        "cells_per_s": pairs * m * n / dur_s,
-        "valu_fp64_tflops": flops / dur_s / 1e12,
     }
 
 
@@ -383,7 +359,7 @@ def workloads_leg(args, dev):
     K64 = oracle.linear_gram(X.double().cpu().numpy(), Y.double().cpu().numpy(), 2)
     K32ref = oracle.linear_gram(X.cpu().numpy(), Y.cpu().numpy(), 2)  # float32 reference semantics
     g = {}
-    for mode in ("ref", "fp32", "fp16x2", "bf16x3", "fp64"):
+    for mode in ("ref", "fp32", "fp16x2", "bf16x3", "fp64", "fp16x2f"):
         k2 = sig.SigPDE(lin, 2, precision=mode)
         k2.gram(X, Y)
         torch.cuda.synchronize()
@@ -392,9 +368,18 @@ def workloads_leg(args, dev):
         g[mode] = {"ms": ms, "pairs_per_s": 128 * 128 / (ms / 1e3), "rel_err_fp64": _rel(Kg, K64)}
         if mode == "ref":
             g[mode]["rel_err_vs_fp32_reference_semantics"] = _rel(Kg, K32ref.astype(np.float64))
+            # the same reference scheme through tiles of one-pass increments (the route taken
+            # past the fused static-kernel Gram's limits)
+            tile = int(np.ceil(np.sqrt(1000)))
+            k2._gram(X, Y, None, None, tile)
+            torch.cuda.synchronize()
+            _, ms_t = _event_ms(lambda: k2._gram(X, Y, None, None, tile), 3)
+            g[mode]["route"] = "fused static-kernel Gram (sigpde_static_gram)"
+            g[mode]["increment_tiles_ms"] = ms_t
     out["gram_128x128_L100_d8_o2_f32"] = g
 
-    # the same Gram with the non-linear static kernels: one-pass increments vs the torch route
+    # the same Gram with the non-linear static kernels: the fused static-kernel Gram, tiles of
+    # one-pass increments, and the torch route
     class _TorchRoute:
         """Hides the static kernel's one-pass increment hooks (the reference's torch route)."""
         def __init__(self, k):
@@ -407,17 +392,22 @@ def workloads_leg(args, dev):
             return self.k.gram(X, Y)
 
     Xs64, Ys64 = X[:16].double().cpu().numpy(), Y[:16].double().cpu().numpy()
+    tile = int(np.ceil(np.sqrt(1000)))
     for kind, K in (("rbf", sig.kernels.RBFKernel(1.0)), ("l1rbf", sig.kernels.L1RBFKernel(1.0))):
         k1 = sig.SigPDE(K, 2)
         k2 = sig.SigPDE(_TorchRoute(K), 2)
         k1.gram(X, Y)
+        k1._gram(X, Y, None, None, tile)
         k2.gram(X, Y)
         torch.cuda.synchronize()
         Kg, ms1 = _event_ms(lambda: k1.gram(X, Y), 3)
+        _, ms_t = _event_ms(lambda: k1._gram(X, Y, None, None, tile), 3)
         _, ms2 = _event_ms(lambda: k2.gram(X, Y), 3)
         ref = oracle.sig_gram(Xs64, Ys64, 2, kind, 1.0)
         out[f"{kind}_gram_128x128_L100_d8_o2_f32"] = {
-            "one_pass_increments_ms": ms1, "torch_route_ms": ms2,
+            "fused_ms": ms1, "pairs_per_s": 128 * 128 / (ms1 / 1e3),
+            "route": "fused static-kernel Gram (sigpde_static_gram)",
+            "one_pass_increment_tiles_ms": ms_t, "torch_route_ms": ms2,
             "rel_err_fp64_16x16": _rel(Kg[:16, :16].double().cpu().numpy(), ref)}
 
     # robust, float64, order 1
@@ -551,8 +541,8 @@ def main():
         ("plain_fp32", "fp32", args.steps, True,
          "precision='fp32': fused, fp32 MFMA increments, float32 wavefront"),
         ("reference_scheme_fp32", "ref", args.ref_steps, True,
-         "precision='ref': torch increment tiles (point Gram, 4-term difference) + generic "
-         "wavefront"),
+         "precision='ref': the reference scheme in float32, fused (point Gram on fp32 MFMA, "
+         "4-term second difference and the float32-typed wavefront in one launch)"),
         ("mixed_bf16x3", "bf16x3", args.steps, True,
          "precision='bf16x3': three-part bf16 increments, six bf16 MFMA products, double "
          "wavefront"),
@@ -570,8 +560,7 @@ def main():
             Ko, mso, _ = timed(args, X, Y, prec, steps, 1)
             others[key] = (Ko, mso, path)
 
-    roof = (fused_roofline(args, X, Y, ms) if args.mode != "ref"
-            else dominant_kernel_roofline(args, X, Y))
+    roof = fused_roofline(args, X, Y, ms)
     line = {
         "metric": "Gram pairs/sec vs plain FP32 at \u22641e-5 rel err (FP64 oracle); p50/p99 "
                   "latency",

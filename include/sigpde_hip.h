@@ -197,6 +197,41 @@ int sigpde_linear_increments(int dtype, const void *ip, int64_t A, int64_t B, in
                              const int64_t *ip_strides, double kscale, int order, void *out,
                              void *stream);
 
+/*
+ * Reference-scheme Gram of a static kernel with the increments formed inside the solve (no
+ * increment tensor). Replaces SigPDE._gram / _gram_sym (sigpde/torch/kernel.py:74-171): the point
+ * values of LinearKernel / RBFKernel / L1RBFKernel (sigpde/torch/static_kernels.py:3-54), their
+ * 4-term second difference / 4^order (kernel.py:18-21) and the Gram solvers
+ * (sigpde/cuda_gram_kernels.py:8-160, scaled: c = inc * (sx[i] * sy[j]) in dtype), in one launch.
+ *   kernel: SIGPDE_STATIC_LINEAR (param = scale), SIGPDE_STATIC_RBF or SIGPDE_STATIC_L1RBF
+ *     (param = sigma); point values in dtype as the reference's torch expressions evaluate them
+ *     (x / sigma as x * (1 / sigma)); inner products and distance sums in another summation order
+ *     than torch's (equal when exact).
+ *   X (Bx, Lx, d), Y (By, Ly, d) C-contiguous; out[i * ldo + j] = K(X_i, Y_j); symmetric != 0
+ *     (Y == X) solves j <= i and mirrors. sx, sy: (Bx,), (By,) scales or NULL.
+ *   Limits: dyadic order <= 3; d <= 64 (float32) / 32 (float64) for the linear and RBF kernels,
+ *     d <= 16 (float32) / 8 (float64) for L1-RBF; refined columns whose lines fit one wave's LDS. Outside them the call
+ *     returns SIGPDE_EINVAL (sigpde_static_gram_supported says so beforehand); the increments
+ *     entry points above plus sigpde_pde_gram take every shape.
+ *   workspace: sigpde_static_workspace_bytes(...) bytes of device memory (same_paths: Y == X).
+ */
+enum sigpde_static_kernel {
+    SIGPDE_STATIC_LINEAR = 0,
+    SIGPDE_STATIC_RBF = 1,
+    SIGPDE_STATIC_L1RBF = 2
+};
+
+int sigpde_static_gram_supported(int dtype, int kernel, int64_t Lx, int64_t Ly, int64_t d,
+                                 int order);
+
+size_t sigpde_static_workspace_bytes(int dtype, int kernel, int64_t Bx, int64_t Lx, int64_t By,
+                                     int64_t Ly, int64_t d, int same_paths);
+
+int sigpde_static_gram(int dtype, int kernel, const void *X, int64_t Bx, int64_t Lx,
+                       const void *Y, int64_t By, int64_t Ly, int64_t d, double param, int order,
+                       const void *sx, const void *sy, int symmetric, void *out, int64_t ldo,
+                       void *workspace, size_t workspace_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

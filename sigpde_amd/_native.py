@@ -52,6 +52,9 @@ _SIGNATURES = {
                                 _vp, _vp],
     "sigpde_linear_increments": [_int, _vp, _i64, _i64, _i64, _i64, ctypes.POINTER(_i64), _dbl,
                                  _int, _vp, _vp],
+    "sigpde_static_gram": [_int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _dbl, _int, _vp,
+                           _vp, _int, _vp, _i64, _vp, ctypes.c_size_t, _vp],
+    "sigpde_static_gram_supported": [_int, _int, _i64, _i64, _i64, _int],
 }
 
 
@@ -77,6 +80,9 @@ def load(path: str | None = None) -> ctypes.CDLL:
             fn.restype = _int
         lib.sigpde_linear_workspace_bytes.restype = ctypes.c_size_t
         lib.sigpde_linear_workspace_bytes.argtypes = [_int, _i64, _i64, _i64, _i64, _i64, _int]
+        lib.sigpde_static_workspace_bytes.restype = ctypes.c_size_t
+        lib.sigpde_static_workspace_bytes.argtypes = [_int, _int, _i64, _i64, _i64, _i64, _i64,
+                                                      _int]
         lib.sigpde_last_error.restype = ctypes.c_char_p
         lib.sigpde_last_error.argtypes = []
         lib.sigpde_version.restype = ctypes.c_char_p
@@ -272,6 +278,46 @@ def linear_pairwise(X: torch.Tensor, Y: torch.Tensor, order: int, kscale: float,
                                   float(kscale), int(order), _ptr(sx), _ptr(sy), _ptr(out),
                                   _ptr(ws), nbytes, _stream(X.device))
     _check(rc, "sigpde_linear_pairwise")
+
+
+STATIC_LINEAR = 0
+STATIC_RBF = 1
+STATIC_L1RBF = 2
+
+
+def static_gram_supported(kind: int, dtype: torch.dtype, Lx: int, Ly: int, d: int,
+                          order: int) -> bool:
+    """True when sigpde_static_gram takes this shape (dyadic order, dim and column limits)."""
+    if dtype not in (torch.float32, torch.float64):
+        return False
+    return bool(load().sigpde_static_gram_supported(dtype_code(dtype), int(kind), int(Lx),
+                                                    int(Ly), int(d), int(order)))
+
+
+def static_gram(kind: int, X: torch.Tensor, Y: torch.Tensor, order: int, param: float,
+                out: torch.Tensor, sx=None, sy=None, symmetric: bool = False) -> None:
+    """Reference-scheme Gram of a static kernel (STATIC_LINEAR: param = scale; STATIC_RBF,
+    STATIC_L1RBF: param = sigma) with the increments formed inside the solve."""
+    _require_gpu(X, Y, out, sx, sy)
+    _require(X.dim() == 3 and Y.dim() == 3 and X.is_contiguous() and Y.is_contiguous()
+             and X.shape[2] == Y.shape[2],
+             "static_gram: X and Y must be contiguous (batch, length, dim) tensors")
+    _require(out.dim() == 2 and out.stride(1) == 1 and tuple(out.shape) == (X.shape[0], Y.shape[0])
+             and out.dtype == X.dtype,
+             "static_gram: out must be (Bx, By) of X's dtype with unit column stride")
+    _require_vec(sx, X.shape[0], "static_gram: sx")
+    _require_vec(sy, Y.shape[0], "static_gram: sy")
+    Bx, Lx, d = X.shape
+    By, Ly, _ = Y.shape
+    same = X.data_ptr() == Y.data_ptr() and X.shape == Y.shape
+    L = load()
+    code = dtype_code(X.dtype)
+    nbytes = L.sigpde_static_workspace_bytes(code, int(kind), Bx, Lx, By, Ly, d, int(same))
+    ws = _workspace(max(nbytes, 1), X.device)
+    rc = L.sigpde_static_gram(code, int(kind), _ptr(X), Bx, Lx, _ptr(Y), By, Ly, d, float(param),
+                              int(order), _ptr(sx), _ptr(sy), int(symmetric), _ptr(out),
+                              out.stride(0), _ptr(ws), nbytes, _stream(X.device))
+    _check(rc, "sigpde_static_gram")
 
 
 def rbf_increments(ip: torch.Tensor, ip_strides, A: int, B: int, m: int, n: int,

@@ -35,6 +35,10 @@ SOURCES = {
     "fused_linear.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
     "fused_pairwise.hip": [],
     "static_increments.hip": [],
+    "fused_point.hip": [],
+    "fused_point_f32.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+    "fused_point_f64.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+    "fused_point_l1.hip": [],
 }
 
 

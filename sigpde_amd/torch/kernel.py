@@ -129,7 +129,13 @@ class SigPDE:
     precision (an addition to the reference's signature; per instance, overridable per call):
       "ref"    -- the reference's arithmetic: increments from the static kernel's point Gram
                   (4-term second difference, in the inputs' dtype), float32 storage with double
-                  per-cell arithmetic for float32 inputs, all-double for float64 inputs;
+                  per-cell arithmetic for float32 inputs, all-double for float64 inputs. Grams of
+                  the built-in static kernels (linear, RBF, L1-RBF) within dyadic order <= 3 and
+                  the dim / length limits of sigpde_static_gram form the point values and their
+                  second difference inside the solve (no increment tensor; inner products and
+                  distance sums in the kernel's summation order, equal to torch's when exact);
+                  other Grams and every pairwise call run the reference's tiles of torch
+                  increments;
       "fp16x2" -- linear kernel only: increments <dx, dy> formed from the paths and multiplied as
                   per-path-scaled fp16 hi/lo pairs on MFMA (fp32 accumulate), double wavefront;
                   relative error ~3e-7 against the float64 reference scheme;
@@ -189,10 +195,46 @@ class SigPDE:
         kscale = _fused_scale(self.static_kernel, precision)
         if kscale is not None:
             return self._gram_fused(x, y, x_scale, y_scale, kscale, precision)
+        spec = self._static_route(x, y)
+        if spec is not None:
+            return self._gram_static(spec, x, y, x_scale, y_scale)
         tile = math.ceil(math.sqrt(int(max_batch)))
         if y is None:
             return self._gram_sym(x, x_scale, tile)
         return self._gram(x, y, x_scale, y_scale, tile)
+
+    def _static_route(self, x, y):
+        """(static kernel code, parameter) when precision "ref" takes the fused static-kernel
+        Gram (the point values and their second difference formed inside the solve), else None:
+        built-in static kernels within the route's dyadic order, dim and length limits."""
+        spec = getattr(self.static_kernel, "fused_static", None)
+        if spec is None or not isinstance(x, torch.Tensor) or not x.is_cuda:
+            return None
+        if y is not None and (not isinstance(y, torch.Tensor) or y.dim() != 3 or x.dim() != 3
+                              or y.shape[2] != x.shape[2]):
+            return None
+        if x.dim() != 3:
+            return None
+        Ly = x.shape[1] if y is None else y.shape[1]
+        if not _native.static_gram_supported(spec[0], x.dtype, x.shape[1], Ly, x.shape[2],
+                                             self.dyadic_order):
+            return None
+        return spec
+
+    def _gram_static(self, spec, x, y, x_scale, y_scale):
+        symmetric = y is None
+        if symmetric:
+            y, y_scale = x, x_scale
+        _check_pair(x, y)
+        Bx, By = x.shape[0], y.shape[0]
+        scaled = x_scale is not None or y_scale is not None
+        sx = _scale_vector(x_scale, Bx, x) if scaled else None
+        sy = (sx if symmetric else _scale_vector(y_scale, By, y)) if scaled else None
+        x = x.contiguous()
+        y = x if symmetric else y.contiguous()
+        out = torch.zeros((Bx, By), device=x.device, dtype=x.dtype)
+        _native.static_gram(spec[0], x, y, self.dyadic_order, spec[1], out, sx, sy, symmetric)
+        return out
 
     def _gram_fused(self, x, y, x_scale, y_scale, kscale, precision):
         symmetric = y is None

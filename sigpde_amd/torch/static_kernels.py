@@ -10,6 +10,9 @@ static kernels. The formulas are the reference's (they define the API's numbers)
   L1-RBF  exp(-sum_k |x_k - y_k| / sigma)
 LinearKernel additionally advertises `fused_linear_scale`, which lets SigPDE route it to the
 fused increment-Gram kernels instead of materialising the point Gram. All three advertise
+`fused_static`, which lets SigPDE.gram in precision "ref" form the point values and their second
+difference inside the solve (sigpde_static_gram), with no increment tensor at all. All three also
+advertise
 `pairwise_increments` / `gram_increments`: the increments of the API's second difference
 (sigpde/torch/kernel.py:13-21) computed in one HIP pass from the point inner products (linear and
 RBF: the same float operations as the torch formulas below, so the same increments) or from the
@@ -30,6 +33,14 @@ class LinearKernel:
     @property
     def fused_linear_scale(self) -> float:
         return float(self.scale)
+
+    @property
+    def fused_static(self):
+        """(static kernel code, scale) for the fused reference-scheme Gram, or None when a subclass
+        overrides gram."""
+        if type(self).gram is not LinearKernel.gram:
+            return None
+        return _native.STATIC_LINEAR, float(self.scale)
 
     def pairwise(self, X: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
         return self.scale * torch.bmm(X, Y.transpose(1, 2).contiguous())
@@ -71,6 +82,14 @@ class LinearKernel:
 class RBFKernel:
     def __init__(self, sigma: float = 1.0):
         self.sigma = sigma
+
+    @property
+    def fused_static(self):
+        """(static kernel code, sigma) for the fused reference-scheme Gram, or None when a
+        subclass overrides gram."""
+        if type(self).gram is not RBFKernel.gram:
+            return None
+        return _native.STATIC_RBF, float(self.sigma)
 
     def pairwise(self, X: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
         A, M, _ = X.shape
@@ -125,6 +144,14 @@ class RBFKernel:
 class L1RBFKernel:
     def __init__(self, sigma: float = 1.0):
         self.sigma = sigma
+
+    @property
+    def fused_static(self):
+        """(static kernel code, sigma) for the fused reference-scheme Gram, or None when a
+        subclass overrides gram."""
+        if type(self).gram is not L1RBFKernel.gram:
+            return None
+        return _native.STATIC_L1RBF, float(self.sigma)
 
     def pairwise(self, X: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
         dist = (X[:, :, None, :] - Y[:, None, :, :]).abs().sum(dim=-1)
