@@ -92,11 +92,31 @@ int sigpde_pde_gram(int dtype, const void *inc, int64_t bx, int64_t by, int64_t 
  *   lambda_out : (B,) the root, set to 0 where knorm < 1 (sigpde/torch/kernel.py:195)
  *   iters_out : (B,) int32 wavefront evaluations of the root-find, or NULL
  */
-enum sigpde_normalizer { SIGPDE_NORM_LOG = 0, SIGPDE_NORM_GIVEN = 1 };
+enum sigpde_normalizer {
+    SIGPDE_NORM_LOG = 0,
+    SIGPDE_NORM_GIVEN = 1,
+    SIGPDE_NORM_TENSOR = 2,  /* built-in, sigpde_robust_norm_builtin */
+    SIGPDE_NORM_CLIP = 3     /* built-in, sigpde_robust_norm_builtin */
+};
 
 int sigpde_robust_norm(int dtype, const void *inc, int64_t B, int64_t m, int64_t inc_stride_b,
                        int order, int normalizer, const void *target, void *knorm, int maxit,
                        double tol, void *lambda_out, int32_t *iters_out, void *stream);
+
+/*
+ * The same root-find with a built-in normaliser evaluated on the device between the K(x,x) solve
+ * and the root-find (no host round trip; the reference takes any Python callable there,
+ * sigpde/torch/kernel.py:180-195, which sigpde_robust_norm serves with SIGPDE_NORM_GIVEN):
+ *   SIGPDE_NORM_LOG    2 - 1/(1 + log x), the reference's default (p0, p1 unused);
+ *   SIGPDE_NORM_TENSOR x for x <= C, else C + C^(1+a) (C^-a - x^-a) / a with C = p0 >= 1,
+ *                      a = p1 > 0 (increasing, at most x, bounded by C (1 + 1/a));
+ *   SIGPDE_NORM_CLIP   min(x, C) with C = p0 >= 1.
+ * Evaluated in dtype, x = K(x,x) after NaN -> +inf; knorm (B,) receives K(x,x).
+ */
+int sigpde_robust_norm_builtin(int dtype, const void *inc, int64_t B, int64_t m,
+                               int64_t inc_stride_b, int order, int normalizer, double p0,
+                               double p1, void *knorm, int maxit, double tol, void *lambda_out,
+                               int32_t *iters_out, void *stream);
 
 /*
  * Fused linear static kernel (k(x, y) = kscale * <x, y>, sigpde/torch/static_kernels.py:3-11):

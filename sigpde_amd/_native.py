@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsigpde_hip.so")
 
 F32, F64 = 0, 1
-NORM_LOG, NORM_GIVEN = 0, 1
+NORM_LOG, NORM_GIVEN, NORM_TENSOR, NORM_CLIP = 0, 1, 2, 3
 PREC_FP16X2 = 1
 PREC_FP32 = 2
 PREC_BF16X3 = 3
@@ -55,6 +55,8 @@ _SIGNATURES = {
     "sigpde_static_gram": [_int, _int, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _dbl, _int, _vp,
                            _vp, _int, _vp, _i64, _vp, ctypes.c_size_t, _vp],
     "sigpde_static_gram_supported": [_int, _int, _i64, _i64, _i64, _int],
+    "sigpde_robust_norm_builtin": [_int, _vp, _i64, _i64, _i64, _int, _int, _dbl, _dbl, _vp, _int,
+                                   _dbl, _vp, _vp, _vp],
 }
 
 
@@ -189,6 +191,25 @@ def robust_norm(inc: torch.Tensor, order: int, lam: torch.Tensor, knorm: torch.T
                                    int(order), mode, _ptr(target), _ptr(knorm), int(maxit),
                                    float(tol), _ptr(lam), _ptr(iters), _stream(inc.device))
     _check(rc, "sigpde_robust_norm")
+
+
+def robust_norm_builtin(inc: torch.Tensor, order: int, lam: torch.Tensor, knorm: torch.Tensor,
+                        normalizer: int, p0: float = 0.0, p1: float = 0.0, maxit: int = 100,
+                        tol: float = 1e-8, iters=None) -> None:
+    """Chandrupatla normalisation per path with a built-in normaliser evaluated on the device
+    (NORM_LOG, NORM_TENSOR with C = p0, a = p1, NORM_CLIP with C = p0); knorm receives K(x,x)."""
+    _require_gpu(inc, lam, knorm, iters)
+    _require(inc.dim() == 3 and inc.shape[1] == inc.shape[2] and inc.stride(2) == 1
+             and inc.stride(1) == inc.shape[2],
+             "robust_norm: increments must be (B, m, m) with unit-stride rows packed m x m")
+    for t, name in ((lam, "lambda"), (knorm, "knorm"), (iters, "iters")):
+        _require_vec(t, inc.shape[0], f"robust_norm: {name}")
+    B, m, _ = inc.shape
+    rc = load().sigpde_robust_norm_builtin(dtype_code(inc.dtype), _ptr(inc), B, m, inc.stride(0),
+                                           int(order), int(normalizer), float(p0), float(p1),
+                                           _ptr(knorm), int(maxit), float(tol), _ptr(lam),
+                                           _ptr(iters), _stream(inc.device))
+    _check(rc, "sigpde_robust_norm_builtin")
 
 
 def linear_workspace_bytes(precision: str, Bx: int, Lx: int, By: int, Ly: int, d: int,

@@ -188,6 +188,7 @@ struct RobustArgs {
   T *knorm;
   T *lambda;
   int32_t *iters;
+  double np0, np1;  // built-in normaliser parameters
 };
 
 template <typename T>
@@ -199,6 +200,25 @@ __device__ __forceinline__ float log_normalizer(float x) {
 template <>
 __device__ __forceinline__ double log_normalizer(double x) {
   return 2.0 - 1.0 / (1.0 + log(x));
+}
+
+// Target of a built-in normaliser for K(x,x) = x (NaN already mapped to +inf), in T:
+//   SIGPDE_NORM_LOG:    2 - 1/(1 + log x), the reference's default (sigpde/torch/kernel.py:23-24);
+//   SIGPDE_NORM_TENSOR: x for x <= C, else C + C^(1+a) (C^-a - x^-a) / a (C = p0, a = p1 > 0):
+//                       increasing, at most x, bounded by C (1 + 1/a);
+//   SIGPDE_NORM_CLIP:   min(x, C) (C = p0).
+template <typename T>
+__device__ __forceinline__ T builtin_target(int id, T x, double p0, double p1) {
+  if (id == SIGPDE_NORM_TENSOR) {
+    const T c = (T)p0, a = (T)p1;
+    if (x <= c) return x;
+    return c + pow(c, (T)1 + a) * (pow(c, -a) - pow(x, -a)) / a;
+  }
+  if (id == SIGPDE_NORM_CLIP) {
+    const T c = (T)p0;
+    return x < c ? x : c;
+  }
+  return log_normalizer<T>(x);
 }
 
 // Chandrupatla's bracketing iteration for one path (sigpde/cuda_pairwise_kernels.py:93-166):
@@ -307,14 +327,14 @@ __global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a, int64_t 
   }
 
   T K1 = (T)1, Tt = (T)1;
-  if (a.normalizer == 0) {
+  if (a.normalizer != SIGPDE_NORM_GIVEN) {
     // _norm_factors (sigpde/torch/kernel.py:181-184): K(x,x), NaN -> inf, target
     double r = 0.0;
     solve_wave<RefArith<T>::kPlain, G, R, GL>(
         P, g, 1.0, smem_raw, wave_lines<RefArith<T>::kPlain, GL>(gl, g.N), [&](int, double v) { r = v; });
     K1 = (T)group_bcast<G>(r, rM);
     if (isnan(K1)) K1 = (T)INFINITY;
-    Tt = log_normalizer<T>(K1);
+    Tt = builtin_target<T>(a.normalizer, K1, a.np0, a.np1);
   } else if (live) {
     K1 = a.knorm[b];
     Tt = a.target[b];
@@ -335,7 +355,7 @@ __global__ __launch_bounds__(kWave) void k_robust_norm(RobustArgs<T> a, int64_t 
   if (live && (lane & (G - 1)) == 0) {
     // scales[norms < 1] = 0 (sigpde/torch/kernel.py:195)
     a.lambda[b] = (K1 < (T)1) ? (T)0 : (T)rf.outv;
-    if (a.normalizer == 0) a.knorm[b] = K1;
+    if (a.normalizer != SIGPDE_NORM_GIVEN) a.knorm[b] = K1;
     if (a.iters) a.iters[b] = rf.it;
   }
 }
@@ -352,10 +372,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(
   const MwGeom g(a.m, a.m, a.order, W, R);
   const T *inc = a.inc + b * a.stride_b;
   T K1 = (T)1, Tt = (T)1;
-  if (a.normalizer == 0) {
+  if (a.normalizer != SIGPDE_NORM_GIVEN) {
     K1 = (T)mw_solve<RefArith<T>::kPlain, W, R>(inc, a.m, (T)1, false, g, 1.0, smem_raw);
     if (isnan(K1)) K1 = (T)INFINITY;
-    Tt = log_normalizer<T>(K1);
+    Tt = builtin_target<T>(a.normalizer, K1, a.np0, a.np1);
   } else {
     K1 = a.knorm[b];
     Tt = a.target[b];
@@ -371,7 +391,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(
   }
   if (threadIdx.x == 0) {
     a.lambda[b] = (K1 < (T)1) ? (T)0 : (T)rf.outv;
-    if (a.normalizer == 0) a.knorm[b] = K1;
+    if (a.normalizer != SIGPDE_NORM_GIVEN) a.knorm[b] = K1;
     if (a.iters) a.iters[b] = rf.it;
   }
 }
@@ -640,6 +660,43 @@ extern "C" int sigpde_pde_gram(int dtype, const void *inc, int64_t bx, int64_t b
   return hip_status(e, "pde_gram launch");
 }
 
+extern "C" int sigpde_robust_norm_builtin(int dtype, const void *inc, int64_t B, int64_t m,
+                                          int64_t inc_stride_b, int order, int normalizer,
+                                          double p0, double p1, void *knorm, int maxit,
+                                          double tol, void *lambda_out, int32_t *iters_out,
+                                          void *stream) {
+  clear_error();
+  if (B == 0) return SIGPDE_OK;
+  if (B < 0 || !inc || !lambda_out || !knorm)
+    return set_error(SIGPDE_EINVAL, "robust_norm: bad batch or null pointer");
+  if (normalizer != SIGPDE_NORM_LOG && normalizer != SIGPDE_NORM_TENSOR &&
+      normalizer != SIGPDE_NORM_CLIP)
+    return set_error(SIGPDE_EINVAL, "robust_norm_builtin: unknown built-in normalizer id %d",
+                     normalizer);
+  if (normalizer == SIGPDE_NORM_TENSOR && !(p0 >= 1.0 && p1 > 0.0 && p0 < INFINITY && p1 < INFINITY))
+    return set_error(SIGPDE_EINVAL, "robust_norm_builtin: SIGPDE_NORM_TENSOR needs C >= 1 and "
+                     "a > 0, got C %g, a %g", p0, p1);
+  if (normalizer == SIGPDE_NORM_CLIP && !(p0 >= 1.0))
+    return set_error(SIGPDE_EINVAL, "robust_norm_builtin: SIGPDE_NORM_CLIP needs C >= 1, got %g",
+                     p0);
+  if (maxit < 0) return set_error(SIGPDE_EINVAL, "robust_norm: maxit must be >= 0");
+  if (int rc = check_grid(dtype, m, m, order)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e;
+  if (dtype == SIGPDE_F32) {
+    RobustArgs<float> a{(const float *)inc, B, inc_stride_b, (int)m, order, normalizer, maxit, tol,
+                        nullptr, (float *)knorm, (float *)lambda_out, iters_out, p0, p1};
+    e = launch_robust(a, st);
+  } else if (dtype == SIGPDE_F64) {
+    RobustArgs<double> a{(const double *)inc, B, inc_stride_b, (int)m, order, normalizer, maxit,
+                         tol, nullptr, (double *)knorm, (double *)lambda_out, iters_out, p0, p1};
+    e = launch_robust(a, st);
+  } else {
+    return set_error(SIGPDE_EINVAL, "robust_norm: unknown dtype %d", dtype);
+  }
+  return hip_status(e, "robust_norm launch");
+}
+
 extern "C" int sigpde_robust_norm(int dtype, const void *inc, int64_t B, int64_t m,
                                   int64_t inc_stride_b, int order, int normalizer,
                                   const void *target, void *knorm, int maxit, double tol,
@@ -649,20 +706,25 @@ extern "C" int sigpde_robust_norm(int dtype, const void *inc, int64_t B, int64_t
   if (B < 0 || !inc || !lambda_out || !knorm)
     return set_error(SIGPDE_EINVAL, "robust_norm: bad batch or null pointer");
   if (normalizer != SIGPDE_NORM_LOG && normalizer != SIGPDE_NORM_GIVEN)
-    return set_error(SIGPDE_EINVAL, "robust_norm: unknown normalizer id %d", normalizer);
-  if (normalizer == SIGPDE_NORM_GIVEN && !target)
-    return set_error(SIGPDE_EINVAL, "robust_norm: SIGPDE_NORM_GIVEN needs target");
+    return set_error(SIGPDE_EINVAL, "robust_norm: unknown normalizer id %d (the other built-in "
+                     "normalisers take parameters: sigpde_robust_norm_builtin)", normalizer);
+  if (normalizer == SIGPDE_NORM_LOG)
+    return sigpde_robust_norm_builtin(dtype, inc, B, m, inc_stride_b, order, normalizer, 0.0, 0.0,
+                                      knorm, maxit, tol, lambda_out, iters_out, stream);
+  if (!target) return set_error(SIGPDE_EINVAL, "robust_norm: SIGPDE_NORM_GIVEN needs target");
   if (maxit < 0) return set_error(SIGPDE_EINVAL, "robust_norm: maxit must be >= 0");
   if (int rc = check_grid(dtype, m, m, order)) return rc;
   hipStream_t st = (hipStream_t)stream;
   hipError_t e;
   if (dtype == SIGPDE_F32) {
     RobustArgs<float> a{(const float *)inc, B, inc_stride_b, (int)m, order, normalizer, maxit, tol,
-                        (const float *)target, (float *)knorm, (float *)lambda_out, iters_out};
+                        (const float *)target, (float *)knorm, (float *)lambda_out, iters_out,
+                        0.0, 0.0};
     e = launch_robust(a, st);
   } else if (dtype == SIGPDE_F64) {
-    RobustArgs<double> a{(const double *)inc, B, inc_stride_b, (int)m, order, normalizer, maxit, tol,
-                         (const double *)target, (double *)knorm, (double *)lambda_out, iters_out};
+    RobustArgs<double> a{(const double *)inc, B, inc_stride_b, (int)m, order, normalizer, maxit,
+                         tol, (const double *)target, (double *)knorm, (double *)lambda_out,
+                         iters_out, 0.0, 0.0};
     e = launch_robust(a, st);
   } else {
     return set_error(SIGPDE_EINVAL, "robust_norm: unknown dtype %d", dtype);

@@ -306,9 +306,17 @@ class RobustSigPDE:
         self.precision = precision
 
     def _norm_factors(self, inc, lam, knorm, normalizer, tol, maxit, iters=None):
-        """sigpde/torch/kernel.py:180-195 on one chunk of self-increments (B, m, m)."""
+        """sigpde/torch/kernel.py:180-195 on one chunk of self-increments (B, m, m). The default
+        log normaliser and the built-in ones (normalizers.py: objects with `device_normalizer`)
+        run inside the root-find kernel; any other callable between its two phases, on the host
+        side, as in the reference."""
         if normalizer is log_normalizer:
             _native.robust_norm(inc, self.dyadic_order, lam, knorm, None, maxit, tol, iters)
+            return
+        builtin = getattr(normalizer, "device_normalizer", None)
+        if builtin is not None:
+            _native.robust_norm_builtin(inc, self.dyadic_order, lam, knorm, builtin[0], builtin[1],
+                                        builtin[2], maxit, tol, iters)
             return
         _native.pde_pairwise(inc, self.dyadic_order, knorm)
         knorm[torch.isnan(knorm)] = torch.inf

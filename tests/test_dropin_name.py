@@ -10,7 +10,7 @@ def test_import_sigpde_torch_as_sig():
     k = sig.kernels.LinearKernel()
     assert k.scale == 1.0
     assert sig.kernels.RBFKernel().sigma == 1.0 and sig.kernels.L1RBFKernel(2.0).sigma == 2.0
-    assert sorted(sig.__all__) == ["RobustSigPDE", "SigPDE", "kernels"]
+    assert sorted(sig.__all__) == ["RobustSigPDE", "SigPDE", "kernels", "normalizers"]
 
 
 def test_reference_submodule_paths():
