@@ -26,6 +26,8 @@ Differences (documented, deliberate):
     x and y have equal lengths.
   * symmetric Gram tiles lying entirely above the diagonal are skipped instead of computing and
     discarding their increments (kernel.py:143-150).
+Tracing: with SIGPDE_ROCTX=1 the public calls and their phases (increments, root-find) run inside
+named roctx ranges (sigpde_amd.trace).
 """
 from __future__ import annotations
 
@@ -35,8 +37,10 @@ from typing import Callable, Optional
 import torch
 
 from .. import _native
+from ..trace import traced
 
 
+@traced("increments.pairwise")
 def pairwise_inner_product(x, y, static_kernel, dyadic_order):
     """Increments (B, Lx-1, Ly-1); sigpde/torch/kernel.py:13-16. Static kernels with a one-pass
     `pairwise_increments` (RBF, L1-RBF) use it."""
@@ -49,6 +53,7 @@ def pairwise_inner_product(x, y, static_kernel, dyadic_order):
     return d / float(2 ** (2 * dyadic_order))
 
 
+@traced("increments.gram")
 def gram_inner_product(x, y, static_kernel, dyadic_order):
     """Increments (Bx, By, Lx-1, Ly-1); sigpde/torch/kernel.py:18-21. Static kernels with a one-pass
     `gram_increments` (RBF, L1-RBF) use it."""
@@ -162,6 +167,7 @@ class SigPDE:
         self.precision = precision
 
     # ------------------------------------------------------------------ pairwise
+    @traced("SigPDE.pairwise")
     def pairwise(self, x, y=None, x_scale=None, y_scale=None, max_batch=1000, max_threads=1024,
                  precision=None):
         y_scale = x_scale if y is None else y_scale
@@ -189,6 +195,7 @@ class SigPDE:
         return out
 
     # ------------------------------------------------------------------ gram
+    @traced("SigPDE.gram")
     def gram(self, x, y=None, x_scale=None, y_scale=None, max_batch=1000, max_threads=1024,
              precision=None):
         precision = precision or self.precision
@@ -305,6 +312,7 @@ class RobustSigPDE:
         self.normalizer = log_normalizer if normalizer is None else normalizer
         self.precision = precision
 
+    @traced("RobustSigPDE.root_find")
     def _norm_factors(self, inc, lam, knorm, normalizer, tol, maxit, iters=None):
         """sigpde/torch/kernel.py:180-195 on one chunk of self-increments (B, m, m). The default
         log normaliser and the built-in ones (normalizers.py: objects with `device_normalizer`)
@@ -335,6 +343,7 @@ class RobustSigPDE:
                                None if iters is None else iters[start:stop])
         return lam
 
+    @traced("RobustSigPDE.normalization")
     def normalization(self, x, normalizer=None, tol=1e-8, maxit=100, max_batch=1000, max_threads=1024):
         normalizer = self.normalizer if normalizer is None else normalizer
         _check_paths(x)
@@ -349,6 +358,7 @@ class RobustSigPDE:
         return (normalizer is log_normalizer and x.shape[1:] == y.shape[1:]
                 and x.dtype == y.dtype and x.device == y.device)
 
+    @traced("RobustSigPDE.pairwise")
     def pairwise(self, x, y=None, normalizer=None, tol=1e-8, maxit=100, max_batch=1000, max_threads=1024):
         symmetric = y is None
         y = x if symmetric else y
@@ -395,6 +405,7 @@ class RobustSigPDE:
             _native.pde_pairwise(inc, self.dyadic_order, out[start:stop], sx, sy)
         return out
 
+    @traced("RobustSigPDE.gram")
     def gram(self, x, y=None, x_scale=None, y_scale=None, normalizer=None, tol=1e-8, maxit=100,
              max_batch=1000, max_threads=1024):
         nz = self.normalizer if normalizer is None else normalizer
