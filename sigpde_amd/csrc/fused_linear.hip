@@ -31,6 +31,7 @@
 #include <climits>
 
 #include "capi_util.h"
+#include "gram16f.h"
 #include "mfma_producer.h"
 
 namespace sigpde {
@@ -143,6 +144,18 @@ __device__ __forceinline__ void gram_unit(const GramFusedArgs<TO> &a, const doub
     if (y0 + ys - 1 > xmax) ys = (int)(xmax - y0 + 1);
   }
   PairGeom<G> geo(a.m, a.n, a.order, ys, R);
+  const int64_t xme = xq * 4 + lane / G;  // local
+  auto write = [&](int sp, double r) {
+    const int64_t y = y0 + sp;
+    if (xme >= bx || y >= by) return;
+    if (symmetric) {
+      if (y > xme) return;
+      out[xme * ldo + y] = (TO)r;
+      out[y * ldo + xme] = (TO)r;
+    } else {
+      out[xme * ldo + y] = (TO)r;
+    }
+  };
   Prod P;
   P.xh = (const E *)a.xh;
   P.xl = (const E *)a.xl;
@@ -166,30 +179,19 @@ __device__ __forceinline__ void gram_unit(const GramFusedArgs<TO> &a, const doub
   }
   P.ybase[0] = yb + y0;
   P.ylane = yb + y0;
-  const int64_t xme = xq * 4 + lane / G;  // local
   {
     const int64_t xc = xb + (xme < bx ? xme : bx - 1);
     P.xsl = Prod::kSplit ? a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0) : 1.0;
     P.pmY = 1.0;
   }
-  ring16_solve<A, R, GL>(P, geo, 1.0, smem_raw, [&](int sp, double r) {
-    const int64_t y = y0 + sp;
-    if (xme >= bx || y >= by) return;
-    if (symmetric) {
-      if (y > xme) return;
-      out[xme * ldo + y] = (TO)r;
-      out[y * ldo + xme] = (TO)r;
-    } else {
-      out[xme * ldo + y] = (TO)r;
-    }
-  }, fused_wave_lines<A, GL>(gl, geo.N));
+  ring16_solve<A, R, GL>(P, geo, 1.0, smem_raw, write, fused_wave_lines<A, GL>(gl, geo.N));
 }
 
 // Occupancy: at dim <= 32 (one K block) the fp16x2 kernel fits three waves per SIMD (measured 9%
 // faster at dim 16); the other instantiations would spill there and keep the compiler's choice.
 // GL: boundary lines in global memory at gl (one block per wave of the launch), else in LDS.
 template <typename TO, int KB, int PREC, int R, bool GL>
-__global__ __launch_bounds__(64)
+__global__ __launch_bounds__(PREC == kPrecFp16x2F ? 128 : 64)
 __attribute__((amdgpu_waves_per_eu(KB == 1 && PREC == kPrecFp16x2 && R == 1 ? 3 : 1))) void
 k_linear_gram_fused(GramFusedArgs<TO> a, const double *sxd, const double *syd, void *gl) {
   const int64_t nq = (a.Bx + 3) / 4;
@@ -226,7 +228,7 @@ struct GramBatch {
 };
 
 template <typename TO, int KB, int PREC, int R, bool GL>
-__global__ __launch_bounds__(64)
+__global__ __launch_bounds__(PREC == kPrecFp16x2F ? 128 : 64)
 __attribute__((amdgpu_waves_per_eu(KB == 1 && PREC == kPrecFp16x2 && R == 1 ? 3 : 1))) void
 k_linear_gram_batch(GramFusedArgs<TO> a, const double *sxd, const double *syd, void *gl,
                     GramBatch b) {
@@ -247,6 +249,115 @@ k_linear_gram_batch(GramFusedArgs<TO> a, const double *sxd, const double *syd, v
   gram_unit<TO, KB, PREC, R, GL>(a, sxd, syd, gl, q.x0, (int64_t)q.x0 + q.bx, u % nq, q.y0,
                                  (int64_t)q.y0 + q.by, u / nq, q.symmetric != 0,
                                  a.out + q.out_off, q.by);
+}
+
+// ------------------------------------------------------------------ fp16x2f Gram (gram16f.h)
+// One work unit of the fp16x2f Gram on a workgroup of NC consumer / producer wave pairs: x quads
+// NC xg .. NC xg + NC - 1 (consumer and producer p take quad NC xg + p) against the ys y paths
+// from yb + yr ys, results as in gram_unit. symmetric: the stream is cut at the group's largest x
+// path (each consumer writes only its y <= x pairs and their mirrors).
+template <typename TO, int KB, int R, int NC>
+__device__ __forceinline__ void gram_unit_f16f(const GramFusedArgs<TO> &a, const double *sxd,
+                                               const double *syd, int64_t xb, int64_t xe,
+                                               int64_t xg, int64_t yb, int64_t ye, int64_t yr,
+                                               bool symmetric, TO *out, int64_t ldo) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int lane = threadIdx.x & 63;
+  const int pi = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) % NC);
+  const int64_t xq = xg * NC + pi;
+  const int64_t bx = xe - xb, by = ye - yb;
+  const int64_t y0 = yr * a.ys;  // local
+  int ys = (int)((by - y0) < a.ys ? (by - y0) : a.ys);
+  if (symmetric) {
+    // only y <= the group's largest x path
+    const int64_t xmax = xg * NC * 4 + 4 * NC - 1 < bx - 1 ? xg * NC * 4 + 4 * NC - 1 : bx - 1;
+    if (y0 > xmax) return;
+    if (y0 + ys - 1 > xmax) ys = (int)(xmax - y0 + 1);
+  }
+  PairGeom<16> geo(a.m, a.n, a.order, ys, R);
+  const int64_t xme = xq * 4 + lane / 16;  // local
+  G16FProducer<KB, NC> Q;
+  Q.xh = (const _Float16 *)a.xh;
+  Q.xl = (const _Float16 *)a.xl;
+  Q.yh = (const _Float16 *)a.yh;
+  Q.yl = (const _Float16 *)a.yl;
+  Q.dpad = a.dpad;
+  Q.m = a.m;
+  Q.n = a.n;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int64_t x = xq * 4 + g;
+    Q.xa[g] = x < bx ? xb + x : -1;
+  }
+  Q.ybase = yb + y0;
+  Q.ny = ye;
+  const int64_t xc = xb + (xme < bx ? xme : bx - 1);
+  const double xsl = a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0);
+  // scale m = kscale 4^-order 2^-(ex+ey) (x_scale y_scale) of stream pair sp of this lane's group
+  auto pair_scale = [&](int sp) -> double {
+    int64_t yp = yb + y0 + sp;
+    yp = yp < ye ? yp : ye - 1;
+    return xsl * (syd ? (double)a.invy[yp] * syd[yp] : (double)a.invy[yp]);
+  };
+  auto write = [&](int sp, double r) {
+    const int64_t y = y0 + sp;
+    if (xme >= bx || y >= by) return;
+    if (symmetric) {
+      if (y > xme) return;
+      out[xme * ldo + y] = (TO)r;
+      out[y * ldo + xme] = (TO)r;
+    } else {
+      out[xme * ldo + y] = (TO)r;
+    }
+  };
+  g16f_solve<R, KB, NC>(Q, geo, smem_raw, pair_scale, write);
+}
+
+// The fp16x2f Gram: one x group (NC x quads) by one y range per workgroup, in the XCD-aware
+// super-tile order of k_linear_gram_fused (tx x groups by ty y ranges).
+template <typename TO, int KB, int R, int NC>
+__global__ __launch_bounds__(128 * NC) void k_linear_gram_f16f(GramFusedArgs<TO> a,
+                                                               const double *sxd,
+                                                               const double *syd) {
+  const int64_t nq = (a.Bx + 3) / 4;
+  const int64_t nxg = (nq + NC - 1) / NC;
+  const int64_t nyr = (a.By + a.ys - 1) / a.ys;
+  const int64_t bid = blockIdx.x;
+  const int64_t per_xcd = gridDim.x / 8;
+  const int64_t lin = (bid % 8) * per_xcd + bid / 8;
+  const int64_t tsz = (int64_t)a.tx * a.ty;
+  const int64_t tile = lin / tsz, w = lin - tile * tsz;
+  const int64_t ntx = (nxg + a.tx - 1) / a.tx;
+  const int64_t xg = (tile % ntx) * a.tx + w % a.tx;
+  const int64_t yr = (tile / ntx) * a.ty + w / a.tx;
+  if (xg >= nxg || yr >= nyr) return;
+  gram_unit_f16f<TO, KB, R, NC>(a, sxd, syd, 0, a.Bx, xg, 0, a.By, yr, a.symmetric != 0, a.out,
+                                a.ldo);
+}
+
+// Several fp16x2f Grams in one launch (k_linear_gram_batch's request table; units are x groups by
+// y ranges, x groups fastest).
+template <typename TO, int KB, int R, int NC>
+__global__ __launch_bounds__(128 * NC) void k_linear_gram_batch_f16f(GramFusedArgs<TO> a,
+                                                                     const double *sxd,
+                                                                     const double *syd,
+                                                                     GramBatch b) {
+  const int64_t bid = blockIdx.x;
+  const int64_t per_xcd = gridDim.x / 8;
+  const int64_t lin = (bid % 8) * per_xcd + bid / 8;
+  if (lin >= b.nunits) return;
+  int lo = 0, hi = b.n - 1;  // the last request whose first unit is <= lin
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (b.r[mid].prefix <= lin) lo = mid;
+    else hi = mid - 1;
+  }
+  const GramBatchReq q = b.r[lo];
+  const int64_t u = lin - q.prefix;
+  const int64_t nxg = ((q.bx + 3) / 4 + NC - 1) / NC;
+  gram_unit_f16f<TO, KB, R, NC>(a, sxd, syd, q.x0, (int64_t)q.x0 + q.bx, u % nxg, q.y0,
+                                (int64_t)q.y0 + q.by, u / nxg, q.symmetric != 0,
+                                a.out + q.out_off, q.by);
 }
 
 // double copies of optional scales
@@ -329,6 +440,64 @@ static hipError_t run_to_double(const void *src, int64_t n, double *dst, hipStre
   return hipGetLastError();
 }
 
+// fp16x2f Gram launch with NC consumer / producer pairs per workgroup (a.ys set; batch as in
+// launch_fused).
+template <typename T, int KB, int R, int NC>
+static hipError_t launch_gram_f16f_nc(GramFusedArgs<T> a, const double *sxd, const double *syd,
+                                      const GramBatch *batch, int N, hipStream_t st) {
+  const int64_t nxg = ((a.Bx + 3) / 4 + NC - 1) / NC;
+  const int64_t nyr = (a.By + a.ys - 1) / a.ys;
+  const int64_t txm = kTileX / NC;  // super-tiles of about kTileX x quads by kTileY y ranges
+  a.tx = (int)(nxg < txm ? nxg : txm);
+  a.ty = (int)(nyr < kTileY ? nyr : kTileY);
+  int64_t nblk = (nxg + a.tx - 1) / a.tx * a.tx * ((nyr + a.ty - 1) / a.ty * a.ty);
+  GramBatch gb{};
+  if (batch) {
+    gb = *batch;
+    int64_t pre = 0;
+    for (int r = 0; r < gb.n; ++r) {
+      gb.r[r].prefix = (int32_t)pre;
+      pre += (((int64_t)gb.r[r].bx + 3) / 4 + NC - 1) / NC * ((gb.r[r].by + a.ys - 1) / a.ys);
+    }
+    gb.nunits = (int32_t)pre;
+    nblk = pre;
+  }
+  // a multiple of 8 blocks so that the XCD-major block order covers every work unit
+  const int64_t grid = (nblk + 7) / 8 * 8;
+  const size_t shmem = g16f_wg_lds_bytes<KB, NC>(N);
+  if (batch)
+    k_linear_gram_batch_f16f<T, KB, R, NC><<<dim3((unsigned)grid), dim3(128 * NC), shmem, st>>>(
+        a, sxd, syd, gb);
+  else
+    k_linear_gram_f16f<T, KB, R, NC><<<dim3((unsigned)grid), dim3(128 * NC), shmem, st>>>(a, sxd,
+                                                                                         syd);
+  return hipGetLastError();
+}
+
+// Consumer waves one CU holds for NC pairs per workgroup (0 when the query fails or the LDS of
+// a workgroup exceeds the CU's).
+template <typename T, int KB, int R, int NC>
+static int f16f_consumers_per_cu(int N) {
+  const size_t lds = g16f_wg_lds_bytes<KB, NC>(N);
+  if (lds > kMaxLdsBytes) return 0;
+  return NC * blocks_per_cu(reinterpret_cast<const void *>(&k_linear_gram_f16f<T, KB, R, NC>),
+                            128 * NC, lds);
+}
+
+// The fp16x2f Gram: NC = 4, 2 or 1 pairs of waves per workgroup, the one that keeps the most
+// consumer waves resident per CU (the larger NC on a tie: a chunk's y operands are fetched once
+// per workgroup); every choice gives the same results.
+template <typename T, int KB, int R>
+static hipError_t launch_gram_f16f(const GramFusedArgs<T> &a, const double *sxd, const double *syd,
+                                   const GramBatch *batch, int N, hipStream_t st) {
+  const int c4 = f16f_consumers_per_cu<T, KB, R, 4>(N);
+  const int c2 = f16f_consumers_per_cu<T, KB, R, 2>(N);
+  const int c1 = f16f_consumers_per_cu<T, KB, R, 1>(N);
+  if (c4 >= c2 && c4 >= c1 && c4 > 0) return launch_gram_f16f_nc<T, KB, R, 4>(a, sxd, syd, batch, N, st);
+  if (c2 >= c1 && c2 > 0) return launch_gram_f16f_nc<T, KB, R, 2>(a, sxd, syd, batch, N, st);
+  return launch_gram_f16f_nc<T, KB, R, 1>(a, sxd, syd, batch, N, st);
+}
+
 // batch (Gram only, may be null): several Grams over the Bx x paths and By y paths (see
 // k_linear_gram_batch); its units and prefixes are filled in here, once the stream length is set.
 template <typename T, int KB, int PREC>
@@ -388,6 +557,14 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
       while (ys > 1 && (int64_t)ys * g1.pair_columns() > (int64_t)1 << 30) ys >>= 1;
     }
     a.ys = ys;
+    if constexpr (PREC == kPrecFp16x2F) {
+      const int N = (int)((Ly - 1) << order);
+      switch (r16_rows(order)) {  // dyadic order <= 2 (check_fused)
+        case 1: return launch_gram_f16f<T, KB, 1>(a, sxd, syd, batch, N, st);
+        case 2: return launch_gram_f16f<T, KB, 2>(a, sxd, syd, batch, N, st);
+        default: return launch_gram_f16f<T, KB, 4>(a, sxd, syd, batch, N, st);
+      }
+    } else {
     const int64_t nyr = (By + ys - 1) / ys;
     // super-tiles of about one XCD's resident waves
     a.tx = (int)(nq < kTileX ? nq : kTileX);
@@ -434,6 +611,7 @@ static hipError_t launch_fused(bool pairwise, const void *X, const void *Y, int6
       case 2: return go(std::integral_constant<int, 2>());
       case 4: return go(std::integral_constant<int, 4>());
       default: return go(std::integral_constant<int, 8>());
+    }
     }
   }
 }

@@ -48,9 +48,13 @@ __global__ __launch_bounds__(64) void k_linear_pairwise_fused(GramFusedArgs<TO> 
     P.xsl = Prod::kSplit ? a.kmult * (double)a.invx[xc] * (sxd ? sxd[xc] : 1.0) : 1.0;
     P.pmY = 1.0;  // the y scale is read when the pair's first chunk is multiplied
   }
-  ring16_solve<A, R, GL>(P, geo, 1.0, smem_raw, [&](int, double r) {
+  auto write = [&](int, double r) {
     if (bme < a.Bx) a.out[bme] = (TO)r;
-  }, fused_wave_lines<A, GL>(gl, geo.N));
+  };
+  if constexpr (A == Arith::MixedF)
+    ring16f_solve<R, GL>(P, geo, smem_raw, write, fused_wave_lines<A, GL>(gl, geo.N));
+  else
+    ring16_solve<A, R, GL>(P, geo, 1.0, smem_raw, write, fused_wave_lines<A, GL>(gl, geo.N));
 }
 
 template <typename T, int KB, int PREC>

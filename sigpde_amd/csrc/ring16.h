@@ -53,35 +53,43 @@ struct R16Bnd {
                                       float>::type;
 };
 
+// Arith::MixedF (the float32 wavefront, ring16f.h) lays its ring out by row instead: ring row r
+// (the consumer lane) holds slots 0..31 at floats r * kR16FRowStride + slot, so that a lane's four
+// columns of a consumer group are two 8-byte LDS reads; its lines carry a one-element shift (reads
+// 16-byte aligned) and a block of kR16FZeros zero floats follows the four lines.
+constexpr int kR16FRowStride = 34;  // 32 slots + 2 pad: 8-byte reads of 16 lanes hit 32 banks
+constexpr int kR16FZeros = 16;
+
 // Boundary line stride (elements): Np + 16 (mirror) plus a pad that puts the four lines of a wave
-// in different LDS banks (the lane groups read and write them in the same instructions).
+// in different LDS banks (the lane groups read and write them in the same instructions). MixedF:
+// Np + 16 (mirror) + 1 (shift) rounded to a multiple of four, so every line starts 16-byte aligned.
 template <Arith A>
 __host__ __device__ inline int r16_line_stride(int Np) {
+  if constexpr (A == Arith::MixedF) return Np + 20;
   return Np + 16 + (sizeof(typename R16Bnd<A>::T) == 4 ? 8 : 4);
-}
-
-// LDS for one wave: four boundary lines, then the ring (the lines' total is a multiple of 128
-// bytes, so the ring stays aligned).
-template <Arith A>
-__host__ __device__ inline size_t r16_lds_bytes(int N) {
-  using RT = typename ArithTraits<A>::RT;
-  using BT = typename R16Bnd<A>::T;
-  const int np = Layout<16>::band_len(N);
-  return (size_t)4 * r16_line_stride<A>(np) * sizeof(BT) +
-         (size_t)kR16Slots * kR16Stride * sizeof(RT);
 }
 
 // LDS of one wave whose boundary lines live in global memory (long rows of columns): the ring.
 template <Arith A>
 __host__ __device__ inline size_t r16_ring_bytes() {
+  if constexpr (A == Arith::MixedF) return (size_t)kWave * kR16FRowStride * sizeof(float);
   return (size_t)kR16Slots * kR16Stride * sizeof(typename ArithTraits<A>::RT);
 }
 
-// Global memory for the four boundary lines of one wave (same layout as in LDS).
+// Global memory for the four boundary lines of one wave (same layout as in LDS; MixedF: and the
+// zero block).
 template <Arith A>
 __host__ __device__ inline size_t r16_line_bytes(int N) {
   using BT = typename R16Bnd<A>::T;
-  return (size_t)4 * r16_line_stride<A>(Layout<16>::band_len(N)) * sizeof(BT);
+  return (size_t)4 * r16_line_stride<A>(Layout<16>::band_len(N)) * sizeof(BT) +
+         (A == Arith::MixedF ? (size_t)kR16FZeros * sizeof(float) : 0);
+}
+
+// LDS for one wave: four boundary lines (MixedF: and the zero block), then the ring (the lines'
+// total is a multiple of 64 bytes, so the ring stays aligned).
+template <Arith A>
+__host__ __device__ inline size_t r16_lds_bytes(int N) {
+  return r16_line_bytes<A>(N) + r16_ring_bytes<A>();
 }
 
 // Chunk writes of a producer. The producing lane l = (c, h), c = l & 15, h = l >> 4, holds element
@@ -112,6 +120,35 @@ struct Ring16Writer {
       if (PAR == 1 && e > 0) a = s1 + e >= kR16Slots ? a - kWrap : a;
 #pragma unroll
       for (int g = 0; g < 4; ++g) *reinterpret_cast<RT *>(rbase + a + g * 16 * sizeof(RT)) = v[g][e];
+    }
+  }
+};
+
+// Chunk writes into the row-major ring of Arith::MixedF: ring row r at floats r * kR16FRowStride,
+// slot (16 (k & 1) + c + (r & 15)) & 31 as above. Element e of pair g sits at row 16g + 4h + e and
+// slot s + e, i.e. e * (kR16FRowStride + 1) floats past element 0; the pairs are 16 rows apart.
+struct Ring16WriterF {
+  uint32_t base[2];  // byte offset of element 0 (pair 0) for even / odd chunks
+  int s1;            // (16 + c + 4h) & 31: the odd chunks' first slot
+  __device__ __forceinline__ void init() {
+    const int lane = threadIdx.x & 63;
+    const int c = lane & 15, h = lane >> 4;
+    const int s0 = c + 4 * h;  // <= 27: even chunks never wrap
+    s1 = (16 + c + 4 * h) & 31;
+    base[0] = (uint32_t)((4 * h * kR16FRowStride + s0) * sizeof(float));
+    base[1] = (uint32_t)((4 * h * kR16FRowStride + s1) * sizeof(float));
+  }
+  template <int PAR>
+  __device__ __forceinline__ void store(float *ring, const float (&v)[4][4]) const {
+    char *rbase = reinterpret_cast<char *>(ring);
+    constexpr uint32_t kWrap = (uint32_t)(kR16Slots * sizeof(float));
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      uint32_t a = base[PAR] + (uint32_t)(e * (kR16FRowStride + 1) * sizeof(float));
+      if (PAR == 1 && e > 0) a = s1 + e >= kR16Slots ? a - kWrap : a;
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+        *reinterpret_cast<float *>(rbase + a + g * 16 * kR16FRowStride * sizeof(float)) = v[g][e];
     }
   }
 };
