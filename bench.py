@@ -58,6 +58,7 @@ from sigpde_amd.synthetic import brownian_paths  # noqa: E402
 
 MI355X_HBM_GBS = 8000.0
 MI355X_FP64_VALU_TFLOPS = 78.6
+MI355X_FP32_VALU_TFLOPS = 157.3
 MI355X_FP16_MFMA_TFLOPS = 2500.0
 MI355X_FP32_MFMA_TFLOPS = 157.3
 MI355X_FP64_MFMA_TFLOPS = 78.6
@@ -67,7 +68,7 @@ FLOPS_PER_CELL = 10  # pde_step as written
 # 256 MiB copy (profiles/gram_fp16x2*_1024x1024_L256_d64_o0_traffic.md); other workloads report
 # null.
 MEASURED_TRAFFIC = {("fp16x2", 1024, 256, 64, 0): 7.718e9,
-                    ("fp16x2f", 1024, 256, 64, 0): 10.815e9}
+                    ("fp16x2f", 1024, 256, 64, 0): 6.011e9}
 
 
 def parse():
@@ -96,7 +97,8 @@ def parse():
     p.add_argument("--serving", type=int, default=300, metavar="N",
                    help="requests of the serving leg (0: skip it)")
     p.add_argument("--serving-sweep", type=int, default=300, metavar="N",
-                   help="requests of the 50%% / 90%% load sweep (the first N of the same set)")
+                   help="requests of the 50%% / 90%% load sweep (the first N of the same set; 0: "
+                        "no sweep)")
     p.add_argument("--serving-rate", type=float, default=None,
                    help="Poisson arrival rate of the serving leg, requests per second (default: "
                         "70%% of the saturated rate measured on the same requests)")
@@ -165,8 +167,9 @@ def fused_roofline(args, X, Y, ms):
                     "float32-typed wavefront, one launch)",
             "fp64": "k_linear_gram_fused<float, 2, 4, 1, false> (fp64 MFMA increments + double "
                     "wavefront, one launch)",
-            "fp16x2f": "k_linear_gram_fused<float, 2, 5, 1, false> (fp16x2 MFMA increments + "
-                       "float32 wavefront, one launch)",
+            "fp16x2f": "k_linear_gram_f16f<float, 2, 1, 4> (fp16x2 MFMA increments on producer "
+                       "waves + float32 wavefront on consumer waves, four pairs of waves per "
+                       "workgroup, one launch)",
             "ref": f"k_point_gram<float, 0, {_point_ne(args.dim)}, {min(1 << args.order, 8)}> "
                    "(fp32 MFMA point Gram, second difference and float32-typed wavefront, one "
                    "launch)"}[args.mode]
@@ -179,30 +182,45 @@ def fused_roofline(args, X, Y, ms):
             ((args.dim + 31) // 32 * 32) * {"fp16x2": 4, "bf16x3": 6, "fp32": 4, "fp64": 8,
                                              "fp16x2f": 4}[args.mode] + \
             pairs * X.element_size()
+    # the wavefront's VALU peak: FP32 cells (fp16x2f, fp32, ref) or FP64 cells
+    f32_wave = args.mode in ("fp16x2f", "fp32", "ref")
+    valu_peak = MI355X_FP32_VALU_TFLOPS if f32_wave else MI355X_FP64_VALU_TFLOPS
+    mfma_peak = {"fp32": MI355X_FP32_MFMA_TFLOPS, "ref": MI355X_FP32_MFMA_TFLOPS,
+                 "fp64": MI355X_FP64_MFMA_TFLOPS}.get(args.mode, MI355X_FP16_MFMA_TFLOPS)
+    # MFMA work as issued: the split modes multiply 3 (fp16x2) or 6 (bf16x3) products over the
+    # dimension padded to the MFMA K
+    dpad = (args.dim + 31) // 32 * 32
+    products = {"fp16x2": 3, "fp16x2f": 3, "bf16x3": 6}.get(args.mode, 1)
+    flops_mfma_issued = flops_inc / args.dim * dpad * products
+    # MFMA and VALU instructions share each SIMD and serialize there (sigpde_issue_rates measures
+    # it): the kernel cannot finish faster than the sum of the two at their peaks
+    t_serial = flops_mfma_issued / (mfma_peak * 1e12) + flops_wave / (valu_peak * 1e12)
     return {
         "kernel": name,
         "bound": "valu",
         "achieved": flops_wave / dur_s / 1e12,
-        "peak": MI355X_FP64_VALU_TFLOPS,
+        "peak": valu_peak,
         "unit": "TFLOP/s",
-        "frac": flops_wave / dur_s / 1e12 / MI355X_FP64_VALU_TFLOPS,
+        "frac": flops_wave / dur_s / 1e12 / valu_peak,
         "traffic": traffic,
         "algorithmic_bytes": alg_bytes,
         "hbm_gbs": None if traffic is None else traffic / dur_s / 1e9,
         "launch_ms": dur_s * 1e3,
         "cells_per_s": cells / dur_s,
         "increment_gram_tflops": flops_inc / dur_s / 1e12,
-        "increment_gram_mfma_frac": flops_inc / dur_s / 1e12 / (
-            {"fp32": MI355X_FP32_MFMA_TFLOPS, "ref": MI355X_FP32_MFMA_TFLOPS,
-             "fp64": MI355X_FP64_MFMA_TFLOPS}.get(
-                args.mode, MI355X_FP16_MFMA_TFLOPS)),
-        "note": "bound: the wavefront's VALU (double wavefront: the cell update is 8 FP64 "
-                "instructions; fp16x2f: 8 FP32 instructions; peak = MI355X FP64 vector 78.6 "
-                "TFLOP/s, which is also the FP32 rate without packed math: the FP32 vector peak "
-                "of 157.3 TFLOP/s needs packed FP32 instructions, which this build turns off); "
-                "wavefront flops = 10 per refined cell (pde_step as written); the increment Gram "
-                "(2*d flops per increment) runs on MFMA beside it; traffic = measured HBM bytes per "
-                "launch, algorithmic_bytes = operands + output",
+        "increment_gram_mfma_frac": flops_inc / dur_s / 1e12 / mfma_peak,
+        "mfma_issued_tflops": flops_mfma_issued / dur_s / 1e12,
+        "serialized_bound_ms": t_serial * 1e3,
+        "frac_serialized": t_serial / dur_s,
+        "note": "bound: the wavefront's VALU (10 flops per refined cell, pde_step as written; "
+                "float32 cells priced at the MI355X FP32 vector peak 157.3 TFLOP/s, which plain "
+                "(unpacked) FP32 FMAs reach: issue_rates measures ~2 ticks per wave64 FMA; double "
+                "cells at the FP64 vector peak 78.6). The increment Gram runs on MFMA on the same "
+                "SIMDs, and MFMA and VALU instructions serialize there (issue_rates: "
+                "mfma_plus_valu_over_sum ~1), so serialized_bound_ms = issued MFMA flops at the "
+                "MFMA peak + wavefront flops at the VALU peak is the kernel's floor and "
+                "frac_serialized its fraction; traffic = measured HBM bytes per launch, "
+                "algorithmic_bytes = operands + output",
     }
 
 
@@ -490,12 +508,14 @@ def serving_leg(args, dev):
     rate = args.serving_rate if args.serving_rate else 0.7 * sat["requests_per_s"]
     out, launches = serve(rate, len(base))
     sweep = []
-    nsw = min(len(base), max(1, args.serving_sweep))
-    if nsw == len(base):
+    nsw = min(len(base), args.serving_sweep)  # 0: no sweep
+    if nsw <= 0:
+        sat_sw = None
+    elif nsw == len(base):
         sat_sw = sat
     else:
         sat_sw, _ = serve(float("inf"), nsw)
-    for frac in (0.5, 0.9):
+    for frac in ((0.5, 0.9) if sat_sw else ()):
         r = frac * sat_sw["requests_per_s"]
         o, _ = serve(r, nsw)
         sweep.append({"load": frac, "requests": nsw, "rate_per_s": r,
@@ -561,6 +581,7 @@ def main():
             others[key] = (Ko, mso, path)
 
     roof = fused_roofline(args, X, Y, ms)
+    roof["issue_rates"] = _native.issue_rates()
     line = {
         "metric": "Gram pairs/sec vs plain FP32 at \u22641e-5 rel err (FP64 oracle); p50/p99 "
                   "latency",

@@ -252,6 +252,17 @@ int sigpde_static_gram(int dtype, int kernel, const void *X, int64_t Bx, int64_t
                        const void *sx, const void *sy, int symmetric, void *out, int64_t ldo,
                        void *workspace, size_t workspace_bytes, void *stream);
 
+/* sigpde_issue_rates: no counterpart in the reference (a measurement helper for the roofline of
+ * the fused kernels, which issue MFMA and VALU work on the same SIMDs). Runs four timing kernels
+ * synchronously on the current device, two one-wave blocks per SIMD of every CU, and writes
+ *   out[0] clock64 ticks per wave64 FP32 FMA per SIMD,   out[1] the same for FP64 FMA,
+ *   out[2] ticks per v_mfma_f32_16x16x32_f16 per SIMD,
+ *   out[3] time of MFMA and FP32 FMA issued by the same waves / (MFMA alone + FMA alone)
+ *          (1: the two serialize on a SIMD, max/sum: they overlap),
+ *   out[4] the device's reported clock in Hz, out[5] CUs, out[6] blocks per probe.
+ * n >= 7. Allocates and frees its own small device buffers. */
+int sigpde_issue_rates(double *out, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -89,12 +89,35 @@ def load(path: str | None = None) -> ctypes.CDLL:
         lib.sigpde_last_error.argtypes = []
         lib.sigpde_version.restype = ctypes.c_char_p
         lib.sigpde_version.argtypes = []
+        lib.sigpde_issue_rates.restype = _int
+        lib.sigpde_issue_rates.argtypes = [ctypes.POINTER(_dbl), _int]
         _lib = lib
         return lib
 
 
 def version() -> str:
     return load().sigpde_version().decode()
+
+
+def issue_rates() -> dict:
+    """Issue rates of the current GPU measured by sigpde_issue_rates (clock64 ticks per wave64
+    instruction per SIMD, two waves per SIMD on every CU), with the vector and MFMA rates they
+    imply at the reported clock."""
+    buf = (_dbl * 7)()
+    _check(load().sigpde_issue_rates(buf, 7), "sigpde_issue_rates")
+    f32, f64, mfma, mix, hz, cus = buf[0], buf[1], buf[2], buf[3], buf[4], buf[5]
+    simds = 4 * cus
+    return {
+        "ticks_per_fp32_fma": f32,
+        "ticks_per_fp64_fma": f64,
+        "ticks_per_mfma_f32_16x16x32_f16": mfma,
+        "mfma_plus_valu_over_sum": mix,
+        "clock_hz": hz,
+        "cus": int(cus),
+        "fp32_valu_tflops": 64 * 2 / f32 * simds * hz / 1e12,
+        "fp64_valu_tflops": 64 * 2 / f64 * simds * hz / 1e12,
+        "fp16_mfma_tflops": 16 * 16 * 32 * 2 / mfma * simds * hz / 1e12,
+    }
 
 
 def _check(rc: int, what: str) -> None:

@@ -39,6 +39,7 @@ SOURCES = {
     "fused_point_f32.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
     "fused_point_f64.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
     "fused_point_l1.hip": [],
+    "rates.hip": [],
 }
 
 
