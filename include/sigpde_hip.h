@@ -40,7 +40,9 @@ enum sigpde_status {
     SIGPDE_OK = 0,
     SIGPDE_EINVAL = 1,   /* bad argument (shape, dtype, null pointer, unsupported size) */
     SIGPDE_EDEVICE = 2,  /* HIP runtime error (launch failure, no device) */
-    SIGPDE_ENOMEM = 3    /* device allocation failed */
+    SIGPDE_ENOMEM = 3,   /* device allocation failed */
+    SIGPDE_ETIMEOUT = 4  /* a bounded wait between the workgroups of a spread pair gave up; the
+                            call's outputs are invalid (sigpde_set_spread) */
 };
 
 /* Message describing the last failure on the calling thread ("" if none). */
@@ -60,6 +62,20 @@ const char *sigpde_version(void);
 int sigpde_pde_pairwise(int dtype, const void *inc, int64_t B, int64_t m, int64_t n,
                         int64_t inc_stride_b, int order, const void *sx, const void *sy,
                         void *out, void *stream);
+/* Few long pairs: when the GPU holds G workgroups per pair at once and G of them shorten a pair's
+ * sequential chain, sigpde_pde_pairwise and the robust root-finds spread each pair over G
+ * workgroups (bands handed between workgroups through global memory, with bounded waits). Such a
+ * call allocates a small stream-ordered workspace and SYNCHRONISES `stream` before it returns (to
+ * check that no wait gave up: SIGPDE_ETIMEOUT); spread calls run one at a time in the process, and
+ * a stream under graph capture never takes the route. Results equal the other routes' bit for bit.
+ *
+ * sigpde_set_spread: no counterpart in the reference (this route's settings, process-wide).
+ *   groups: 0 automatic (default), 1 never spread, 2..32 that many workgroups per pair whenever
+ *           they are all resident (tests use it to run the route on small grids);
+ *   wait_seconds: wall-clock limit of one wait (default 2, at most 600);
+ *   test_hook: 0, or 1 = workgroup 0 of each pair publishes no progress, so the waits on it give
+ *           up (a test of the failure path: the call returns SIGPDE_ETIMEOUT). */
+int sigpde_set_spread(int groups, double wait_seconds, int test_hook);
 
 /*
  * Wavefront from a precomputed increment tile, one PDE per (x, y) pair.

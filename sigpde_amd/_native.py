@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsigpde_hip.so")
 
 F32, F64 = 0, 1
+EINVAL, EDEVICE, ENOMEM, ETIMEOUT = 1, 2, 3, 4
 NORM_LOG, NORM_GIVEN, NORM_TENSOR, NORM_CLIP = 0, 1, 2, 3
 PREC_FP16X2 = 1
 PREC_FP32 = 2
@@ -57,11 +58,16 @@ _SIGNATURES = {
     "sigpde_static_gram_supported": [_int, _int, _i64, _i64, _i64, _int],
     "sigpde_robust_norm_builtin": [_int, _vp, _i64, _i64, _i64, _int, _int, _dbl, _dbl, _vp, _int,
                                    _dbl, _vp, _vp, _vp],
+    "sigpde_set_spread": [_int, _dbl, _int],
 }
 
 
 class NativeError(RuntimeError):
-    """Raised when a C ABI call returns a non-zero status."""
+    """Raised when a C ABI call returns a non-zero status (`status`: the SIGPDE_E* code)."""
+
+    def __init__(self, msg: str, status: int = 0):
+        super().__init__(msg)
+        self.status = status
 
 
 def load(path: str | None = None) -> ctypes.CDLL:
@@ -120,10 +126,19 @@ def issue_rates() -> dict:
     }
 
 
+def set_spread(groups: int = 0, wait_seconds: float = 2.0, test_hook: int = 0) -> None:
+    """Settings of the route that spreads each of few long pairs over several workgroups
+    (process-wide): groups 0 = automatic, 1 = never, 2..32 = that many whenever resident;
+    wait_seconds bounds each wait between workgroups; test_hook 1 makes workgroup 0 of each pair
+    publish no progress, so spread calls fail with ETIMEOUT (a test of the failure path)."""
+    _check(load().sigpde_set_spread(int(groups), float(wait_seconds), int(test_hook)),
+           "sigpde_set_spread")
+
+
 def _check(rc: int, what: str) -> None:
     if rc != 0:
         msg = load().sigpde_last_error().decode()
-        raise NativeError(f"{what} failed (status {rc}): {msg}")
+        raise NativeError(f"{what} failed (status {rc}): {msg}", rc)
 
 
 def dtype_code(t: torch.dtype) -> int:

@@ -25,6 +25,25 @@
 //     row per period.
 // Sequential steps per pair: ((nb - 1) D + P) 16 instead of nb P 16 on one wave
 // (nb = ceil(M / (64 R))).
+//
+// Spread over G workgroups (XW; for fewer pairs than the GPU holds workgroups): the WT = G W
+// waves of a pair's workgroups share its bands as one workgroup's W waves do (band b on wave
+// b mod WT of workgroup (b mod WT) / W; D from WT). Inside a workgroup the handoff stays in LDS;
+// between workgroups (wave W - 1 of workgroup g feeds wave 0 of workgroup g + 1 mod G) it goes
+// through a window in global memory: link l (into workgroup l) has two buffers of wing =
+// 16 (D + 2 + kXwgSlack) entries, used by alternate rounds of WT bands. The workgroups run
+// their periods on their own, each publishing the count of periods it has completed (release,
+// agent scope) at the top of every period, and wait (acquire) only where the schedule needs a
+// neighbour's progress:
+//   * reading (wave 0, band b >= 1, period p): columns up to 16 q + 16 were written by band b - 1
+//     by its period q + 4, so the feeding workgroup must have completed p - D + 5 periods;
+//   * writing (wave W - 1, period p): a slot's earlier value was read at least 5 + kXwgSlack
+//     periods before (the same band's column v - wing, or the round two bands-of-WT before), so
+//     the fed workgroup must have completed p - 4 - kXwgSlack periods.
+// Both needs are below the waiting workgroup's own period, so the workgroup furthest behind never
+// waits: with all of a pair's workgroups resident, every wait ends. Every wait is still bounded
+// (wall clock); one that gives up sets the launch's failure flag, after which no wait waits, the
+// kernel runs to its end, and the host reports the launch as failed.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -47,14 +66,16 @@ struct MwGeom {
   int D;            // periods between band starts
   int win;          // window entries
   int T;            // periods of the whole pair
-  __host__ __device__ MwGeom(int m_, int n_, int o, int W, int R_ = 1)
+  // W: the waves that share the pair's bands (all workgroups of a spread pair); Dmin: the least
+  // band offset (5 covers a band's skew; a pair spread over workgroups takes more as slack)
+  __host__ __device__ MwGeom(int m_, int n_, int o, int W, int R_ = 1, int Dmin = 5)
       : m(m_), n(n_), order(o), R(R_) {
     M = m << o;
     N = n << o;
     nb = (M + 64 * R - 1) / (64 * R);
     P = (N + 63 + 15) / 16;
     const int need = (P + 1 + (W - 2)) / (W - 1);  // ceil((P + 1) / (W - 1))
-    D = need > 5 ? need : 5;
+    D = need > Dmin ? need : Dmin;
     win = 16 * (D + 2);
     T = (nb - 1) * D + P;
   }
@@ -97,6 +118,76 @@ __device__ __forceinline__ void mw_load_rep16(const RT *row, int c0, RT (&dst)[1
                    std::make_integer_sequence<int, 16>());
 }
 
+// Periods of slack in a cross-workgroup window (the writer may run this far ahead of the reader).
+constexpr int kXwgSlack = 8;
+
+// One workgroup's view of a pair spread over G >= 2 workgroups. Every thread holds a copy;
+// seenIn / seenOut / dead are the same on every lane of a wave.
+struct Xwg {
+  void *win;        // G links x 2 buffers x wing entries of the window type
+  int *done;        // G counters: periods each workgroup of the pair has completed
+  int *err;         // the launch's failure flag (nonzero: a wait gave up)
+  long long limit;  // wall-clock ticks one wait may take
+  int G, gi;        // workgroups of the pair, this one's index
+  int wing;         // entries of a global window buffer
+  int hook;         // 1: workgroup 0 publishes no progress (failure-path test)
+  int base;         // periods of the pair's earlier solves (each solve runs T on every workgroup)
+  int seenIn, seenOut;  // the largest counts this wave has synchronised with (acquire)
+  bool dead;
+};
+
+__host__ __device__ inline int xwg_window_entries(int D) { return 16 * (D + 2 + kXwgSlack); }
+
+// Waits until *flag >= need (seen: the largest value this wave has read; it needs no read while
+// it covers need). Acquire, agent scope. Gives up after x.limit wall-clock ticks, or at once when
+// the launch has failed, setting the failure flag; returns false then.
+__device__ inline bool xwg_wait(const int *flag, int need, int &seen, Xwg &x) {
+  if (seen >= need) return true;
+  if (x.dead) return false;
+  const long long t0 = (long long)wall_clock64();
+  for (;;) {
+    const int v = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (v >= need) {
+      seen = v;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      return true;
+    }
+    const int failed = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(x.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (failed || (long long)wall_clock64() - t0 > x.limit) {
+      __hip_atomic_store(x.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      x.dead = true;
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
+// Accesses through pointers of a stated address space (3: LDS, 1: global), so that code choosing
+// between an LDS and a global window keeps LDS and global instructions instead of flat ones.
+template <typename V>
+__device__ __forceinline__ V as_load(const void *p, std::integral_constant<int, 3>) {
+  return *(const __attribute__((address_space(3))) V *)(p);
+}
+template <typename V>
+__device__ __forceinline__ V as_load(const void *p, std::integral_constant<int, 1>) {
+  return *(const __attribute__((address_space(1))) V *)(p);
+}
+template <typename V>
+__device__ __forceinline__ void as_store(void *p, V v, std::integral_constant<int, 3>) {
+  *(__attribute__((address_space(3))) V *)(p) = v;
+}
+template <typename V>
+__device__ __forceinline__ void as_store(void *p, V v, std::integral_constant<int, 1>) {
+  *(__attribute__((address_space(1))) V *)(p) = v;
+}
+
+// The workgroup (of a pair spread over G) whose waves hold the last band, and so K[M][N].
+__host__ __device__ inline int mw_corner_group(const MwGeom &g, int W, int G) {
+  return ((g.nb - 1) % (W * G)) / W;
+}
+
 // LDS of one workgroup: W windows of the band-boundary type, then one double (the result).
 template <Arith A>
 __host__ __device__ inline size_t mw_lds_bytes(const MwGeom &g, int W) {
@@ -110,10 +201,13 @@ __host__ __device__ inline size_t mw_lds_bytes(const MwGeom &g, int W) {
 // R refined rows per lane (g.R == R): lane rho of band b holds rows 64 R b + R rho + 1 .. + R,
 // which share one row of increments, so the coefficients and the shift from the lane above serve
 // R cells a step. Every thread of the workgroup (64 W threads) must call it; all get K[M][N].
-template <Arith A, int W, int R>
+//
+// XW: the pair is spread over xs->G workgroups (g built with W G waves); the result is K[M][N]
+// only in workgroup mw_corner_group(g, W, G).
+template <Arith A, int W, int R, bool XW = false>
 __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, int64_t ld,
                            typename ArithTraits<A>::RT scale, bool scaled, const MwGeom &g,
-                           double s, void *lds) {
+                           double s, void *lds, Xwg *xs = nullptr) {
   using RT = typename ArithTraits<A>::RT;
   using BT = typename R16Bnd<A>::T;
   const int wave = threadIdx.x >> 6;
@@ -123,6 +217,14 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
                                            ((size_t)W * g.win * sizeof(BT) + 7) / 8 * 8);
   BT *inWin = wins + wave * g.win;                    // band b-1 -> this wave's band b
   BT *outWin = wins + ((wave + 1) % W) * g.win;       // this wave's band b -> band b+1
+  int inSz = g.win, outSz = g.win;
+  const BT *gIn = nullptr;  // XW: wave 0's window in global memory (bands >= 1)
+  BT *gOut = nullptr;       // XW: wave W - 1's window in global memory
+  int WT = W, gw = wave;  // waves sharing the pair's bands, this wave's index among them
+  if constexpr (XW) {
+    WT = W * xs->G;
+    gw = xs->gi * W + wave;
+  }
   const int jo_max = g.n - 1;
 
   double k[R], kPrev = 1.0, result = 0.0;
@@ -194,11 +296,18 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
   __syncthreads();
 
   for (int p = 0; p < g.T; ++p) {
-    // this wave's band in period p (at most one: W D >= P + 1)
+    if constexpr (XW) {
+      // the periods before p are complete: wave W - 1's window writes of them (its lane 63
+      // publishes, after its own stores) and wave 0's window reads (done before the barrier)
+      if (p > 0 && wave == W - 1 && lane == 63 && !(xs->hook == 1 && xs->gi == 0))
+        __hip_atomic_store(xs->done + xs->gi, xs->base + p, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // this wave's band in period p (at most one: WT D >= P + 1)
     int b = -1, q = 0;
-    const int rel = p - wave * g.D;
+    const int rel = p - gw * g.D;
     if (rel >= 0) {
-      const int bb = wave + W * (rel / (W * g.D));
+      const int bb = gw + WT * (rel / (WT * g.D));
       const int qq = p - bb * g.D;
       if (bb < g.nb && qq < g.P) {
         b = bb;
@@ -206,6 +315,29 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
       }
     }
     if (b >= 0) {
+      if constexpr (XW) {
+        const int G = xs->G;
+        if (wave == 0) {
+          // band 0 reads the boundary row, ones, from wave 0's LDS window (which no band
+          // writes when the pair is spread); the others read their link
+          gIn = nullptr;
+          inSz = g.win;
+          if (b > 0) {
+            gIn = static_cast<const BT *>(xs->win) +
+                  ((size_t)xs->gi * 2 + ((b / WT) & 1)) * xs->wing;
+            inSz = xs->wing;
+            xwg_wait(xs->done + (xs->gi + G - 1) % G, xs->base + p - g.D + 5, xs->seenIn, *xs);
+          }
+        }
+        if (wave == W - 1) {
+          gOut = static_cast<BT *>(xs->win) +
+                 ((size_t)((xs->gi + 1) % G) * 2 + (((b + 1) / WT) & 1)) * xs->wing;
+          outSz = xs->wing;
+          if (b + 1 < g.nb)
+            xwg_wait(xs->done + (xs->gi + 1) % G, xs->base + p - 4 - kXwgSlack, xs->seenOut,
+                     *xs);
+        }
+      }
       if (q == 0) {
 #pragma unroll
         for (int r = 0; r < R; ++r) k[r] = 1.0;  // k[i][0]
@@ -219,18 +351,36 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
       const int i = 64 * R * b + R * lane + 1;  // this lane's first row (1-based)
       // lane 63 writes its columns 16 q + st - 62 at window slots wb + st + 1 (win is a multiple
       // of 16: only the last may wrap)
-      const int wb = (16 * q) % g.win;
-      const int w15 = wb + 16 < g.win ? wb + 16 : 0;
+      const int wb = (16 * q) % outSz;
+      const int w15 = wb + 16 < outSz ? wb + 16 : 0;
       // lane 0's row above the band for the period's columns 16 q + 1 .. 16 q + 16, all written
       // in earlier periods (D >= 5), at slots r0 .. r0 + 15: read (broadcast to every lane) four
       // at a time, a group of four steps ahead of their use. Slots of columns past N hold stale
       // values; they reach only lane 0's cells past N, which are not computed.
-      const int r0 = (16 * q + 64) % g.win;
+      const int r0 = (16 * q + 64) % inSz;
       auto tops4 = [&](BT (&t)[4], int grp) {
         typedef BT vec4 __attribute__((ext_vector_type(4), aligned(16)));
-        const vec4 v = *reinterpret_cast<const vec4 *>(inWin + r0 + 4 * grp);
+        vec4 v;
+        if constexpr (XW) {
+          if (gIn)
+            v = as_load<vec4>(gIn + r0 + 4 * grp, std::integral_constant<int, 1>());
+          else
+            v = as_load<vec4>(inWin + r0 + 4 * grp, std::integral_constant<int, 3>());
+        } else {
+          v = *reinterpret_cast<const vec4 *>(inWin + r0 + 4 * grp);
+        }
 #pragma unroll
         for (int e = 0; e < 4; ++e) t[e] = v[e];
+      };
+      auto put = [&](int slot, BT val) {
+        if constexpr (XW) {
+          if (gOut)
+            as_store<BT>(gOut + slot, val, std::integral_constant<int, 1>());
+          else
+            as_store<BT>(outWin + slot, val, std::integral_constant<int, 3>());
+        } else {
+          outWin[slot] = val;
+        }
       };
       BT tcur[4], tnxt[4];
       tops4(tcur, 0);
@@ -250,7 +400,7 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
           double cA, cB;
           coefs<A>(cc[st], s, 0.0, 0.0, cA, cB);
           step_rows(up, cA, cB);
-          if (hand) outWin[st < 15 ? wb + st + 1 : w15] = (BT)k[R - 1];
+          if (hand) put(st < 15 ? wb + st + 1 : w15, (BT)k[R - 1]);
           kPrev = up;
         }
       } else {
@@ -268,7 +418,7 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
             double cA, cB;
             coefs<A>(cc[st], s, 0.0, 0.0, cA, cB);
             step_rows(up, cA, cB);
-            if (hand) outWin[st < 15 ? wb + st + 1 : w15] = (BT)k[R - 1];
+            if (hand) put(st < 15 ? wb + st + 1 : w15, (BT)k[R - 1]);
             if (v == g.N) {
 #pragma unroll
               for (int r = 0; r < R; ++r)
@@ -281,8 +431,14 @@ __device__ double mw_solve(const typename ArithTraits<A>::RT *__restrict__ inc, 
     }
     __syncthreads();
   }
+  if constexpr (XW) {
+    if (wave == W - 1 && lane == 63 && !(xs->hook == 1 && xs->gi == 0))
+      __hip_atomic_store(xs->done + xs->gi, xs->base + g.T, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    xs->base += g.T;
+  }
   // the corner lane publishes K[M][N]
-  if ((g.M - 1 - 64 * R * (g.nb - 1)) / R == lane && wave == (g.nb - 1) % W) *res = result;
+  if ((g.M - 1 - 64 * R * (g.nb - 1)) / R == lane && gw == (g.nb - 1) % WT) *res = result;
   __syncthreads();
   const double r = *res;
   __syncthreads();  // the next call may overwrite the slot

@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include "capi_util.h"
+#include <mutex>
 #include <type_traits>
 
 #include "mw_solver.h"
@@ -113,6 +114,55 @@ __global__ __launch_bounds__(64 * W) void k_pde_pairwise_mw(PairwiseArgs<T> a) {
   const double r = mw_solve<RefArith<T>::kPlain, W, R>(a.inc + b * a.stride_b, a.n, scale,
                                                        scaled, g, 1.0, smem_raw);
   if (threadIdx.x == 0) a.out[b] = (T)r;
+}
+
+// Pairs spread over G workgroups each (mw_solve XW). Per pair, ws holds a control block of
+// kXwgCtrlBytes (G progress counters from offset 0, the root-find's result sequence at
+// kXwgSeqOff and result at kXwgResOff) and then the pair's windows; all zero at launch.
+constexpr size_t kXwgCtrlBytes = 256;
+constexpr size_t kXwgSeqOff = 128, kXwgResOff = 136;
+constexpr int kXwgMaxGroups = 32;
+constexpr int kXwgDmin = 7;  // band offset of a spread pair: two periods of slack on the reads
+
+struct XwgLaunch {
+  char *ws;
+  int *err;          // the launch's failure flag
+  size_t pairBytes;  // control block + windows of one pair
+  long long limit;   // wall-clock ticks one wait may take
+  int G, wing, hook;
+};
+
+__device__ inline Xwg xwg_state(const XwgLaunch &x, int64_t pair, int gi) {
+  char *blk = x.ws + pair * x.pairBytes;
+  Xwg s;
+  s.win = blk + kXwgCtrlBytes;
+  s.done = reinterpret_cast<int *>(blk);
+  s.err = x.err;
+  s.limit = x.limit;
+  s.G = x.G;
+  s.gi = gi;
+  s.wing = x.wing;
+  s.hook = x.hook;
+  s.base = 0;
+  s.seenIn = s.seenOut = 0;
+  s.dead = false;
+  return s;
+}
+
+// One pair per G workgroups of W waves: the pair's bands over all G W waves (few long pairs on a
+// GPU with more CUs than pairs). Workgroup gi of pair b is block b G + gi.
+template <typename T, int W, int R>
+__global__ __launch_bounds__(64 * W) void k_pde_pairwise_xw(PairwiseArgs<T> a, XwgLaunch x) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int64_t b = blockIdx.x / x.G;
+  const int gi = (int)(blockIdx.x - b * x.G);
+  const MwGeom g(a.m, a.n, a.order, W * x.G, R, kXwgDmin);
+  const bool scaled = a.sx || a.sy;
+  const T scale = scaled ? (a.sy ? a.sy[b] : (T)1) * (a.sx ? a.sx[b] : (T)1) : (T)1;
+  Xwg xs = xwg_state(x, b, gi);
+  const double r = mw_solve<RefArith<T>::kPlain, W, R, true>(a.inc + b * a.stride_b, a.n, scale,
+                                                             scaled, g, 1.0, smem_raw, &xs);
+  if (gi == mw_corner_group(g, W, x.G) && threadIdx.x == 0) a.out[b] = (T)r;
 }
 
 // Pair p of a Gram tile -> (i, j). Symmetric diagonal tiles (off_x == off_y, bx == by) list only
@@ -396,6 +446,68 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(
   }
 }
 
+// The root-find with each path spread over G workgroups of W waves. Every workgroup of a path
+// runs the same root-find: the corner workgroup publishes each evaluation's K[M][N] (release) and
+// the others take it from there (acquire, bounded wait), so all update the same state the same
+// way and run the same evaluations.
+template <typename T, int W, int R>
+__global__ __launch_bounds__(64 * W) void k_robust_norm_xw(RobustArgs<T> a, XwgLaunch x) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ double shared_result;
+  const int64_t b = blockIdx.x / x.G;
+  const int gi = (int)(blockIdx.x - b * x.G);
+  const MwGeom g(a.m, a.m, a.order, W * x.G, R, kXwgDmin);
+  const T *inc = a.inc + b * a.stride_b;
+  Xwg xs = xwg_state(x, b, gi);
+  char *blk = x.ws + b * x.pairBytes;
+  int *seqp = reinterpret_cast<int *>(blk + kXwgSeqOff);
+  double *resp = reinterpret_cast<double *>(blk + kXwgResOff);
+  const bool corner = gi == mw_corner_group(g, W, x.G);
+  int seq = 0, seen = 0;
+  auto share = [&](double r) -> double {
+    ++seq;
+    if (corner) {
+      if (threadIdx.x == 0) {
+        *resp = r;
+        __hip_atomic_store(seqp, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      return r;
+    }
+    if (threadIdx.x < 64) {
+      xwg_wait(seqp, seq, seen, xs);
+      if (threadIdx.x == 0) shared_result = *resp;
+    }
+    __syncthreads();
+    const double v = shared_result;
+    __syncthreads();
+    return v;
+  };
+  T K1 = (T)1, Tt = (T)1;
+  if (a.normalizer != SIGPDE_NORM_GIVEN) {
+    K1 = (T)share(
+        mw_solve<RefArith<T>::kPlain, W, R, true>(inc, a.m, (T)1, false, g, 1.0, smem_raw, &xs));
+    if (isnan(K1)) K1 = (T)INFINITY;
+    Tt = builtin_target<T>(a.normalizer, K1, a.np0, a.np1);
+  } else {
+    K1 = a.knorm[b];
+    Tt = a.target[b];
+  }
+  RootFind<T> rf;
+  rf.init(K1, Tt);
+  bool active = a.maxit > 0;  // uniform over the path's workgroups
+  while (active) {
+    const double xt = rf.next();
+    const double sol = share(mw_solve<RefArith<T>::kScaled, W, R, true>(
+        inc, a.m, (T)1, false, g, xt * xt, smem_raw, &xs));
+    if (rf.update(xt, sol, a.tol, a.maxit)) active = false;
+  }
+  if (corner && threadIdx.x == 0) {
+    a.lambda[b] = (K1 < (T)1) ? (T)0 : (T)rf.outv;
+    if (a.normalizer != SIGPDE_NORM_GIVEN) a.knorm[b] = K1;
+    if (a.iters) a.iters[b] = rf.it;
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 // Lane groups of 16 (four pairs per wave) need half the LDS per pair and fill the CUs better when
 // there are many pairs; groups of 64 finish a single pair in fewer sequential steps.
@@ -527,8 +639,130 @@ static bool mw_robust(const RobustArgs<T> &a, hipStream_t st, hipError_t &e) {
   return true;
 }
 
+// ------------------------------------------------------------------ spread route
+// Settings of the spread route (sigpde_set_spread): 0 = automatic, 1 = never, G >= 2 = G
+// workgroups per pair whenever they are all resident; the wall-clock limit of one wait; the
+// failure-path test hook. Spread launches run one at a time in the process: each needs all of its
+// workgroups resident at once, which two such launches in flight could deny each other.
+static std::mutex g_xwg_mutex;
+static int g_xwg_groups = 0;
+static double g_xwg_wait_s = 2.0;
+static int g_xwg_hook = 0;
+
+// Workgroups per pair for the spread route over W waves each (0: not taken). The wave chain of a
+// band step is latency-bound, so the spread runs one wave per SIMD (W = 4, one workgroup per CU)
+// and takes about ((P + 1) / kXwgDmin + 1) waves per pair plus one workgroup: enough that D
+// falls to kXwgDmin (the pair's sequential periods, (nb - 1) D + P, at their least). All the
+// launch's workgroups must be resident at once (the waits between them rely on it), at most one
+// per CU, and the sequential periods must fall to at most 4/5 of the single-workgroup route's
+// (whose workgroups hold up to 16 waves). Sets the LDS bytes of a workgroup.
 template <typename T>
-static hipError_t launch_pairwise(const PairwiseArgs<T> &a, hipStream_t st) {
+static int xwg_plan(int64_t pairs, int m, int n, int order, int R, int W, const void *kern,
+                    hipStream_t st, size_t &lds_out) {
+  constexpr Arith A = RefArith<T>::kPlain;
+  if (g_xwg_groups == 1 || pairs < 1) return 0;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+    (void)hipGetLastError();  // the launch that follows reports its own status
+    return 0;
+  }
+  if (cs != hipStreamCaptureStatusNone) return 0;
+  const int forced = g_xwg_groups;
+  int G = forced;
+  const MwGeom g16(m, n, order, 16, R);
+  if (forced < 2) {
+    const int waves = (g16.P + 1 + kXwgDmin - 1) / kXwgDmin + 1;
+    G = std::min((waves + W - 1) / W + 1, kXwgMaxGroups);
+    G = (int)std::min<int64_t>(G, cu_count() / pairs);
+    if (G < 2) return 0;
+  }
+  const MwGeom g(m, n, order, W * G, R, kXwgDmin);
+  const size_t lds = mw_lds_bytes<A>(g, W);
+  if (lds > kMaxLdsBytes) return 0;
+  if (pairs * G > (int64_t)cu_count() * blocks_per_cu(kern, 64 * W, lds)) return 0;
+  if (forced < 2 && (int64_t)g.T * 5 > (int64_t)g16.T * 4) return 0;
+  lds_out = lds;
+  return G;
+}
+
+// Runs launch(XwgLaunch) with the pairs' workspace (zeroed), waits for it, checks its failure flag
+// and frees the workspace. Returns a SIGPDE status.
+template <typename BT, class F>
+static int xwg_run(int64_t pairs, int G, const MwGeom &g, hipStream_t st, const char *what,
+                   F &&launch) {
+  std::lock_guard<std::mutex> lock(g_xwg_mutex);
+  XwgLaunch x{};
+  x.G = G;
+  x.wing = xwg_window_entries(g.D);
+  x.hook = g_xwg_hook;
+  x.pairBytes = kXwgCtrlBytes + ((size_t)G * 2 * x.wing * sizeof(BT) + 255) / 256 * 256;
+  int dev = 0, khz = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+  if (e != hipSuccess) return hip_status(e, what);
+  if (khz <= 0) khz = 100000;  // the constant 100 MHz counter of CDNA GPUs
+  x.limit = (long long)(g_xwg_wait_s * 1e3 * (double)khz);
+  const size_t bytes = 256 + (size_t)pairs * x.pairBytes;
+  void *ws = nullptr;
+  e = hipMallocAsync(&ws, bytes, st);
+  if (e != hipSuccess) return hip_status(e, what);
+  x.err = static_cast<int *>(ws);
+  x.ws = static_cast<char *>(ws) + 256;
+  e = hipMemsetAsync(ws, 0, bytes, st);
+  if (e == hipSuccess) {
+    launch(x);
+    e = hipGetLastError();
+  }
+  int failed = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&failed, ws, sizeof(int), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  const hipError_t ef = hipFreeAsync(ws, st);
+  if (e == hipSuccess) e = ef;
+  if (e != hipSuccess) return hip_status(e, what);
+  if (failed)
+    return set_error(SIGPDE_ETIMEOUT,
+                     "%s: a wait between the %d workgroups of a pair gave up after %g s (were "
+                     "they all resident?); the outputs are invalid",
+                     what, G, g_xwg_wait_s);
+  return SIGPDE_OK;
+}
+
+// The spread route for pairwise solves (float32: 16 waves a workgroup, float64: 8); R as for the
+// multi-wave route. Returns true when taken, with its status in rc.
+constexpr int kXwgWaves = 4;  // waves per workgroup of the spread route
+
+template <typename T, int R>
+static bool xwg_pairwise(const PairwiseArgs<T> &a, hipStream_t st, int &rc) {
+  constexpr int W = kXwgWaves;
+  size_t lds = 0;
+  const int G = xwg_plan<T>(a.B, a.m, a.n, a.order, R, W,
+                            reinterpret_cast<const void *>(&k_pde_pairwise_xw<T, W, R>), st, lds);
+  if (G < 2) return false;
+  const MwGeom g(a.m, a.n, a.order, W * G, R, kXwgDmin);
+  rc = xwg_run<typename R16Bnd<RefArith<T>::kPlain>::T>(
+      a.B, G, g, st, "pde_pairwise (spread)", [&](const XwgLaunch &x) {
+        k_pde_pairwise_xw<T, W, R><<<dim3((unsigned)(a.B * G)), dim3(64 * W), lds, st>>>(a, x);
+      });
+  return true;
+}
+
+template <typename T, int R>
+static bool xwg_robust(const RobustArgs<T> &a, hipStream_t st, int &rc) {
+  constexpr int W = kXwgWaves;
+  size_t lds = 0;
+  const int G = xwg_plan<T>(a.B, a.m, a.m, a.order, R, W,
+                            reinterpret_cast<const void *>(&k_robust_norm_xw<T, W, R>), st, lds);
+  if (G < 2) return false;
+  const MwGeom g(a.m, a.m, a.order, W * G, R, kXwgDmin);
+  rc = xwg_run<typename R16Bnd<RefArith<T>::kPlain>::T>(
+      a.B, G, g, st, "robust_norm (spread)", [&](const XwgLaunch &x) {
+        k_robust_norm_xw<T, W, R><<<dim3((unsigned)(a.B * G)), dim3(64 * W), lds, st>>>(a, x);
+      });
+  return true;
+}
+
+template <typename T>
+static hipError_t launch_pairwise_local(const PairwiseArgs<T> &a, hipStream_t st) {
   constexpr Arith A = RefArith<T>::kPlain;
   hipError_t e = hipSuccess;
   if ((a.order >= 1 && mw_pairwise<T, 2>(a, st, e)) || mw_pairwise<T, 1>(a, st, e)) return e;
@@ -545,6 +779,15 @@ static hipError_t launch_pairwise(const PairwiseArgs<T> &a, hipStream_t st) {
   k_pde_pairwise<T, 64, 1, false><<<dim3((unsigned)a.B), dim3(kWave), lds_for<T, 64>(N), st>>>(
       a, 0, nullptr);
   return hipGetLastError();
+}
+
+// Pairs spread over workgroups when that shortens their sequential chain, else the routes of
+// one workgroup per pair or per wave. Returns a SIGPDE status.
+template <typename T>
+static int launch_pairwise(const PairwiseArgs<T> &a, hipStream_t st) {
+  int rc = SIGPDE_OK;
+  if ((a.order >= 1 && xwg_pairwise<T, 2>(a, st, rc)) || xwg_pairwise<T, 1>(a, st, rc)) return rc;
+  return hip_status(launch_pairwise_local(a, st), "pde_pairwise launch");
 }
 
 template <typename T>
@@ -568,7 +811,7 @@ static hipError_t launch_gram(const GramArgs<T> &a, hipStream_t st) {
 }
 
 template <typename T>
-static hipError_t launch_robust(const RobustArgs<T> &a, hipStream_t st) {
+static hipError_t launch_robust_local(const RobustArgs<T> &a, hipStream_t st) {
   constexpr Arith A = RefArith<T>::kPlain;
   hipError_t e = hipSuccess;
   if ((a.order >= 1 && mw_robust<T, 2>(a, st, e)) || mw_robust<T, 1>(a, st, e)) return e;
@@ -585,6 +828,13 @@ static hipError_t launch_robust(const RobustArgs<T> &a, hipStream_t st) {
   k_robust_norm<T, 64, 1, false><<<dim3((unsigned)a.B), dim3(kWave), lds_for<T, 64>(N), st>>>(
       a, 0, nullptr);
   return hipGetLastError();
+}
+
+template <typename T>
+static int launch_robust(const RobustArgs<T> &a, hipStream_t st) {
+  int rc = SIGPDE_OK;
+  if ((a.order >= 1 && xwg_robust<T, 2>(a, st, rc)) || xwg_robust<T, 1>(a, st, rc)) return rc;
+  return hip_status(launch_robust_local(a, st), "robust_norm launch");
 }
 
 // m: increments along the rows (bands), n: along the columns (boundary lines).
@@ -619,19 +869,17 @@ extern "C" int sigpde_pde_pairwise(int dtype, const void *inc, int64_t B, int64_
   if (B < 0 || !inc || !out) return set_error(SIGPDE_EINVAL, "pde_pairwise: bad batch or null pointer");
   if (int rc = check_grid(dtype, m, n, order)) return rc;
   hipStream_t st = (hipStream_t)stream;
-  hipError_t e;
   if (dtype == SIGPDE_F32) {
     PairwiseArgs<float> a{(const float *)inc, B, inc_stride_b, (int)m, (int)n, order,
                           (const float *)sx, (const float *)sy, (float *)out};
-    e = launch_pairwise(a, st);
-  } else if (dtype == SIGPDE_F64) {
+    return launch_pairwise(a, st);
+  }
+  if (dtype == SIGPDE_F64) {
     PairwiseArgs<double> a{(const double *)inc, B, inc_stride_b, (int)m, (int)n, order,
                            (const double *)sx, (const double *)sy, (double *)out};
-    e = launch_pairwise(a, st);
-  } else {
-    return set_error(SIGPDE_EINVAL, "pde_pairwise: unknown dtype %d", dtype);
+    return launch_pairwise(a, st);
   }
-  return hip_status(e, "pde_pairwise launch");
+  return set_error(SIGPDE_EINVAL, "pde_pairwise: unknown dtype %d", dtype);
 }
 
 extern "C" int sigpde_pde_gram(int dtype, const void *inc, int64_t bx, int64_t by, int64_t m,
@@ -682,19 +930,16 @@ extern "C" int sigpde_robust_norm_builtin(int dtype, const void *inc, int64_t B,
   if (maxit < 0) return set_error(SIGPDE_EINVAL, "robust_norm: maxit must be >= 0");
   if (int rc = check_grid(dtype, m, m, order)) return rc;
   hipStream_t st = (hipStream_t)stream;
-  hipError_t e;
   if (dtype == SIGPDE_F32) {
     RobustArgs<float> a{(const float *)inc, B, inc_stride_b, (int)m, order, normalizer, maxit, tol,
                         nullptr, (float *)knorm, (float *)lambda_out, iters_out, p0, p1};
-    e = launch_robust(a, st);
+    return launch_robust(a, st);
   } else if (dtype == SIGPDE_F64) {
     RobustArgs<double> a{(const double *)inc, B, inc_stride_b, (int)m, order, normalizer, maxit,
                          tol, nullptr, (double *)knorm, (double *)lambda_out, iters_out, p0, p1};
-    e = launch_robust(a, st);
-  } else {
-    return set_error(SIGPDE_EINVAL, "robust_norm: unknown dtype %d", dtype);
+    return launch_robust(a, st);
   }
-  return hip_status(e, "robust_norm launch");
+  return set_error(SIGPDE_EINVAL, "robust_norm: unknown dtype %d", dtype);
 }
 
 extern "C" int sigpde_robust_norm(int dtype, const void *inc, int64_t B, int64_t m,
@@ -715,19 +960,31 @@ extern "C" int sigpde_robust_norm(int dtype, const void *inc, int64_t B, int64_t
   if (maxit < 0) return set_error(SIGPDE_EINVAL, "robust_norm: maxit must be >= 0");
   if (int rc = check_grid(dtype, m, m, order)) return rc;
   hipStream_t st = (hipStream_t)stream;
-  hipError_t e;
   if (dtype == SIGPDE_F32) {
     RobustArgs<float> a{(const float *)inc, B, inc_stride_b, (int)m, order, normalizer, maxit, tol,
                         (const float *)target, (float *)knorm, (float *)lambda_out, iters_out,
                         0.0, 0.0};
-    e = launch_robust(a, st);
+    return launch_robust(a, st);
   } else if (dtype == SIGPDE_F64) {
     RobustArgs<double> a{(const double *)inc, B, inc_stride_b, (int)m, order, normalizer, maxit,
                          tol, (const double *)target, (double *)knorm, (double *)lambda_out,
                          iters_out, 0.0, 0.0};
-    e = launch_robust(a, st);
-  } else {
-    return set_error(SIGPDE_EINVAL, "robust_norm: unknown dtype %d", dtype);
+    return launch_robust(a, st);
   }
-  return hip_status(e, "robust_norm launch");
+  return set_error(SIGPDE_EINVAL, "robust_norm: unknown dtype %d", dtype);
+}
+
+extern "C" int sigpde_set_spread(int groups, double wait_seconds, int test_hook) {
+  clear_error();
+  if (groups < 0 || groups > kXwgMaxGroups)
+    return set_error(SIGPDE_EINVAL, "set_spread: groups must be 0 (automatic), 1 (never) or 2..%d",
+                     kXwgMaxGroups);
+  if (!(wait_seconds > 0.0 && wait_seconds <= 600.0))
+    return set_error(SIGPDE_EINVAL, "set_spread: wait_seconds must be in (0, 600]");
+  if (test_hook < 0 || test_hook > 1) return set_error(SIGPDE_EINVAL, "set_spread: test_hook is 0 or 1");
+  std::lock_guard<std::mutex> lock(g_xwg_mutex);
+  g_xwg_groups = groups;
+  g_xwg_wait_s = wait_seconds;
+  g_xwg_hook = test_hook;
+  return SIGPDE_OK;
 }

@@ -65,6 +65,20 @@ def test_argument_validation_without_device(lib):
     assert L.sigpde_pde_pairwise(0, None, 0, 5, 5, 25, 0, None, None, None, None) == 0
 
 
+def test_spread_settings_without_device(lib):
+    """sigpde_set_spread validates its settings and touches no device."""
+    L = _native.load()
+    assert L.sigpde_set_spread(-1, 2.0, 0) == 1 and "groups" in L.sigpde_last_error().decode()
+    assert L.sigpde_set_spread(33, 2.0, 0) == 1
+    assert L.sigpde_set_spread(0, 0.0, 0) == 1 and "wait_seconds" in L.sigpde_last_error().decode()
+    assert L.sigpde_set_spread(0, 2.0, 2) == 1 and "test_hook" in L.sigpde_last_error().decode()
+    with pytest.raises(_native.NativeError) as ei:
+        _native.set_spread(1, -1.0)
+    assert ei.value.status == _native.EINVAL
+    for g in (1, 4, 32, 0):
+        assert L.sigpde_set_spread(g, 2.0, 0) == 0
+
+
 def test_fused_entry_points_empty_and_negative_batches_without_device(lib):
     """The fused linear entry points return at once for an empty result, with null pointers (what
     empty device tensors carry), and reject a negative batch; neither touches the device."""
