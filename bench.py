@@ -192,9 +192,14 @@ def fused_roofline(args, X, Y, ms):
     dpad = (args.dim + 31) // 32 * 32
     products = {"fp16x2": 3, "fp16x2f": 3, "bf16x3": 6}.get(args.mode, 1)
     flops_mfma_issued = flops_inc / args.dim * dpad * products
-    # MFMA and VALU instructions share each SIMD and serialize there (sigpde_issue_rates measures
-    # it): the kernel cannot finish faster than the sum of the two at their peaks
-    t_serial = flops_mfma_issued / (mfma_peak * 1e12) + flops_wave / (valu_peak * 1e12)
+    # MFMA and VALU instructions share each SIMD: issued by the same wave they take the sum of
+    # their times; issued by different waves they overlap in part (sigpde_issue_rates measures
+    # both). The kernel cannot finish faster than the slower of the two at its peak
+    # (overlap_bound_ms), and with no overlap not faster than their sum (serialized_bound_ms).
+    t_mfma = flops_mfma_issued / (mfma_peak * 1e12)
+    t_valu = flops_wave / (valu_peak * 1e12)
+    t_serial = t_mfma + t_valu
+    t_overlap = max(t_mfma, t_valu)
     return {
         "kernel": name,
         "bound": "valu",
@@ -210,17 +215,24 @@ def fused_roofline(args, X, Y, ms):
         "increment_gram_tflops": flops_inc / dur_s / 1e12,
         "increment_gram_mfma_frac": flops_inc / dur_s / 1e12 / mfma_peak,
         "mfma_issued_tflops": flops_mfma_issued / dur_s / 1e12,
+        "mfma_bound_ms": t_mfma * 1e3,
+        "valu_bound_ms": t_valu * 1e3,
         "serialized_bound_ms": t_serial * 1e3,
         "frac_serialized": t_serial / dur_s,
+        "overlap_bound_ms": t_overlap * 1e3,
+        "frac_overlap": t_overlap / dur_s,
         "note": "bound: the wavefront's VALU (10 flops per refined cell, pde_step as written; "
                 "float32 cells priced at the MI355X FP32 vector peak 157.3 TFLOP/s, which plain "
                 "(unpacked) FP32 FMAs reach: issue_rates measures ~2 ticks per wave64 FMA; double "
                 "cells at the FP64 vector peak 78.6). The increment Gram runs on MFMA on the same "
-                "SIMDs, and MFMA and VALU instructions serialize there (issue_rates: "
-                "mfma_plus_valu_over_sum ~1), so serialized_bound_ms = issued MFMA flops at the "
-                "MFMA peak + wavefront flops at the VALU peak is the kernel's floor and "
-                "frac_serialized its fraction; traffic = measured HBM bytes per launch, "
-                "algorithmic_bytes = operands + output",
+                "SIMDs: MFMA and VALU instructions of one wave take the sum of their times "
+                "(issue_rates: mfma_plus_valu_over_sum ~1), those of different waves overlap in "
+                "part (split_1x1_over_sum, split_2x2_over_sum against the full-overlap value "
+                "split_full_overlap_over_sum), so the kernel's floor lies between "
+                "overlap_bound_ms = max(issued MFMA flops at the MFMA peak, wavefront flops at the "
+                "VALU peak) and serialized_bound_ms = their sum; frac_overlap and "
+                "frac_serialized are the launch's fractions of them; traffic = measured HBM bytes "
+                "per launch, algorithmic_bytes = operands + output",
     }
 
 

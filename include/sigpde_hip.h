@@ -269,13 +269,21 @@ int sigpde_static_gram(int dtype, int kernel, const void *X, int64_t Bx, int64_t
                        void *workspace, size_t workspace_bytes, void *stream);
 
 /* sigpde_issue_rates: no counterpart in the reference (a measurement helper for the roofline of
- * the fused kernels, which issue MFMA and VALU work on the same SIMDs). Runs four timing kernels
- * synchronously on the current device, two one-wave blocks per SIMD of every CU, and writes
+ * the fused kernels, which issue MFMA and VALU work on the same SIMDs). Runs timing kernels
+ * synchronously on the current device and writes
  *   out[0] clock64 ticks per wave64 FP32 FMA per SIMD,   out[1] the same for FP64 FMA,
- *   out[2] ticks per v_mfma_f32_16x16x32_f16 per SIMD,
+ *   out[2] ticks per v_mfma_f32_16x16x32_f16 per SIMD (these three: two one-wave blocks per
+ *          SIMD of every CU),
  *   out[3] time of MFMA and FP32 FMA issued by the same waves / (MFMA alone + FMA alone)
- *          (1: the two serialize on a SIMD, max/sum: they overlap),
- *   out[4] the device's reported clock in Hz, out[5] CUs, out[6] blocks per probe.
+ *          (1: the two serialize, max/sum: they overlap),
+ *   out[4] the device's reported clock in Hz, out[5] CUs, out[6] blocks per rate probe;
+ * with n >= 12 also (waves of one role each, MFMA only or FMA only, on the same SIMDs)
+ *   out[7] one MFMA wave beside one FMA wave per SIMD: the slower role's time / (the two roles'
+ *          times alone), out[8] the full-overlap value of that ratio (max / sum of the times
+ *          alone), out[9] the ratio with two waves of each role per SIMD,
+ *   out[10] ticks per FP32 add with a DPP row_shr:1 operand per SIMD (two waves per SIMD),
+ *   out[11] dependent-issue latency of FP32 FMA in ticks (one wave per SIMD, one chain),
+ *   out[12] (n >= 13) the same for the DPP add.
  * n >= 7. Allocates and frees its own small device buffers. */
 int sigpde_issue_rates(double *out, int n);
 

@@ -108,9 +108,10 @@ def version() -> str:
 def issue_rates() -> dict:
     """Issue rates of the current GPU measured by sigpde_issue_rates (clock64 ticks per wave64
     instruction per SIMD, two waves per SIMD on every CU), with the vector and MFMA rates they
-    imply at the reported clock."""
-    buf = (_dbl * 7)()
-    _check(load().sigpde_issue_rates(buf, 7), "sigpde_issue_rates")
+    imply at the reported clock, how far MFMA and FP32 VALU work of different waves on one SIMD
+    overlap, the rate of an FP32 add with a DPP operand and two dependent-issue latencies."""
+    buf = (_dbl * 13)()
+    _check(load().sigpde_issue_rates(buf, 13), "sigpde_issue_rates")
     f32, f64, mfma, mix, hz, cus = buf[0], buf[1], buf[2], buf[3], buf[4], buf[5]
     simds = 4 * cus
     return {
@@ -118,6 +119,12 @@ def issue_rates() -> dict:
         "ticks_per_fp64_fma": f64,
         "ticks_per_mfma_f32_16x16x32_f16": mfma,
         "mfma_plus_valu_over_sum": mix,
+        "split_1x1_over_sum": buf[7],
+        "split_full_overlap_over_sum": buf[8],
+        "split_2x2_over_sum": buf[9],
+        "ticks_per_fp32_add_dpp": buf[10],
+        "latency_fp32_fma_ticks": buf[11],
+        "latency_fp32_add_dpp_ticks": buf[12],
         "clock_hz": hz,
         "cus": int(cus),
         "fp32_valu_tflops": 64 * 2 / f32 * simds * hz / 1e12,

@@ -15,6 +15,14 @@ def test_issue_rates_are_plausible():
     assert 1.0 <= r["ticks_per_fp64_fma"] <= 16.0, r
     assert 8.0 <= r["ticks_per_mfma_f32_16x16x32_f16"] <= 64.0, r
     assert 0.3 <= r["mfma_plus_valu_over_sum"] <= 2.0, r
+    # waves of one role each: the slower role takes at least its time alone (the full-overlap
+    # ratio) and, give or take timing noise, at most the sum of the two
+    assert 0.3 <= r["split_full_overlap_over_sum"] <= 1.0, r
+    assert r["split_full_overlap_over_sum"] * 0.9 <= r["split_1x1_over_sum"] <= 1.3, r
+    assert 0.3 <= r["split_2x2_over_sum"] <= 1.3, r
+    assert 1.0 <= r["ticks_per_fp32_add_dpp"] <= 16.0, r
+    assert 2.0 <= r["latency_fp32_fma_ticks"] <= 64.0, r
+    assert 2.0 <= r["latency_fp32_add_dpp_ticks"] <= 64.0, r
     assert r["fp32_valu_tflops"] <= 157.3 * 1.1, r
     assert r["fp64_valu_tflops"] <= 78.6 * 1.1, r
     assert r["fp16_mfma_tflops"] <= 2500 * 1.1, r
