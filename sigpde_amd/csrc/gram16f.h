@@ -56,9 +56,15 @@ __host__ __device__ inline int g16f_prow(int L) {
   return (L & 48) | ((9 * (K + H)) & 15);
 }
 
-// LDS of one consumer: four boundary lines (r16_line_stride<MixedF>), the zero block, the ring.
+// Floats of a consumer's sink: the line writes of lanes other than a group's last land there, at
+// lane + 0 .. 15, so the writes need no branch on the lane.
+constexpr int kG16FSink = 80;
+
+// LDS of one consumer: four boundary lines (r16_line_stride<MixedF>), the zero block, the ring,
+// the sink.
 __host__ __device__ inline size_t g16f_lds_bytes(int N) {
-  return r16_line_bytes<Arith::MixedF>(N) + (size_t)kWave * kG16FRowStride * sizeof(float);
+  return r16_line_bytes<Arith::MixedF>(N) +
+         ((size_t)kWave * kG16FRowStride + kG16FSink) * sizeof(float);
 }
 
 // ------------------------------------------------------------------ producer waves
@@ -212,6 +218,8 @@ __host__ __device__ inline size_t g16f_wg_lds_bytes(int N) {
 // A producer wave's loop. Chunk j is loaded (each producer its share) in window j - 4, staged in
 // window j - 3, multiplied by every producer in window j - 2 and written to its consumer's ring in
 // window j - 1 (window i: after beta_(i-1)); the first chunks are loaded whole by every producer.
+// A window stages the loads of the previous window first and then issues its own, so a load has
+// the rest of its window (the ring writes and the multiplications) to arrive.
 // A staging slot holds chunk j from window j - 3 to window j - 2, and chunk j + 2 is staged in
 // window j - 1, after the barrier that ends the reads of chunk j. nperiods + 1 barriers, as many
 // as each consumer's. The ring starts zeroed (slots of virtual columns before the stream are read
@@ -245,13 +253,13 @@ __device__ void g16f_produce(G16FProducer<KB, NC> &P, const PairGeom<16> &geo, f
   for (int i = 0; i <= nperiods; ++i) {
     __syncthreads();  // beta_(i-1)
     const int k = i + 1;  // acc holds chunk k
-    if (k < nchunks) P.store(rbytes, ph);
     if (k + 2 < nchunks) P.stage(stage + ((k + 2) & 1) * kSlot);  // chunk i + 3
+    if (k + 3 < nchunks) P.fetchB_share(geo);                       // chunk i + 4
+    if (k < nchunks) P.store(rbytes, ph);
     if (k + 1 < nchunks) {                                          // chunk i + 2
       P.unstage(stage + ((k + 1) & 1) * kSlot);
       P.compute(geo, k + 2 < nchunks);
     }
-    if (k + 3 < nchunks) P.fetchB_share(geo);                       // chunk i + 4
     ph = ph == 2 ? 0 : ph + 1;
   }
 }
@@ -295,21 +303,21 @@ __device__ __forceinline__ void g16f_group(const char *rb, const R16FPeriod &pp,
     }
     vals[k] = st.k[R - 1];
   }
-  if (rho == 15) {
+  if constexpr (!F::kReset) {
+    // every lane writes, without a branch: a group's last lane into its line, the others into
+    // their sink slots (pp.w)
     float *w = pp.w + 4 * I;
-    if constexpr (F::kOnes) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) w[k] = 1.0f;
-    } else if constexpr (F::kReset) {
+    for (int k = 0; k < 4; ++k) w[k] = F::kOnes ? 1.0f : vals[k];
+  } else if (rho == 15) {
+    // reset / capture periods: the line (ones after a pair's last band) and, in periods that
+    // start a band, its mirror
+    float *w = pp.w + 4 * I;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const float v = pp.ones ? 1.0f : vals[k];
-        w[k] = v;
-        if (pp.wm) pp.wm[4 * I + k] = v;
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) w[k] = vals[k];
+    for (int k = 0; k < 4; ++k) {
+      const float v = pp.ones ? 1.0f : vals[k];
+      w[k] = v;
+      if (pp.wm) pp.wm[4 * I + k] = v;
     }
   }
 }
@@ -337,6 +345,7 @@ __device__ void g16f_consume(const PairGeom<16> &g, float *bnd, float *ring, Sca
   float *line = bnd + (lane >> 4) * bs;
   const float *zeros = bnd + 4 * bs;
   const char *rb0 = reinterpret_cast<const char *>(ring + g16f_prow(lane) * kG16FRowStride);
+  float *sink = ring + kWave * kG16FRowStride;  // kG16FSink floats after the ring
 
   const int nchunks = g.columns() / kChunk;
   const int nperiods = nchunks + 1;  // the last lane finishes at step columns + 14
@@ -371,7 +380,7 @@ __device__ void g16f_consume(const PairGeom<16> &g, float *bnd, float *ring, Sca
     R16FPeriod pp;
     pp.z = zbase + zstep * rst0;
     pp.zNext = zbase + zstep * rn;
-    pp.w = line + rst0 + 1;
+    pp.w = rho == 15 ? line + rst0 + 1 : sink + lane;  // reset periods: only rho 15 writes
     pp.wm = reset ? line + Np + 1 : nullptr;
     pp.ones = reset ? band == 0 : band == g.nb - 1;
     const bool cap = sp < g.stream && (tM >> 4) == per;
