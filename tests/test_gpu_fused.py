@@ -202,3 +202,17 @@ def test_fp16x2f_refuses_grids_past_512_refined_cells_a_side():
     got = k1.gram(cuda(x)).cpu().numpy().astype(np.float64)
     ref = O.linear_gram(x.astype(np.float64), x.astype(np.float64), 1)
     check(got, ref, "fp16x2f")
+
+
+@pytest.mark.parametrize("order,L,d", [(0, 256, 64), (1, 129, 16), (2, 65, 8), (0, 40, 100)])
+def test_fp16x2f_wave_pair_gram_equals_single_wave_pairwise(order, L, d):
+    """The fp16x2f Gram (consumer / producer wave pairs sharing y operands) and the fp16x2f
+    pairwise call (one wave per four pairs) form the same increment tiles in the same MFMA order
+    and run the same float32 cell arithmetic, so the Gram's diagonal equals the pairwise results
+    bit for bit."""
+    x = brownian_np(21, L, d, 1 / L, 31, np.float32)
+    y = brownian_np(21, L, d, 1 / L, 32, np.float32)
+    k = sig.SigPDE(sig.kernels.LinearKernel(0.9), order, precision="fp16x2f")
+    K = k.gram(cuda(x), cuda(y))
+    p = k.pairwise(cuda(x), cuda(y))
+    assert torch.equal(torch.diagonal(K), p)
