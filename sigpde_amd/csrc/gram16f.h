@@ -90,20 +90,22 @@ struct G16FProducer {
   float4v acc[4];                      // the last chunk multiplied, one 16 x 16 tile per pair
   int64_t yrow;                        // first element of the y path being fetched
   Cursor<G> cB, cC;                    // next chunk to fetch / to multiply
-  uint32_t woff[4];                    // byte offset of element e's slot at phase 0 (pair 0)
-  uint32_t wwrap;                      // bit e: element e's slot wraps at phase 2
+  uint32_t woff[4];                    // LDS address of element e's slot at phase 0 (pair 0)
+  uint32_t woff2[4];                   // the same at phase 2 (wrapped past the ring's last slot)
 
   __device__ __forceinline__ uint32_t kslice() const { return 8 * ((threadIdx.x & 63) >> 4); }
 
-  __device__ __forceinline__ void init_writer() {
+  // ring: the consumer's ring in LDS (its LDS address goes into woff / woff2)
+  __device__ __forceinline__ void init_writer(const float *ring) {
     const int lane = threadIdx.x & 63;
     const int c = lane & 15, h = lane >> 4;
-    wwrap = 0;
+    const uint32_t rb = (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) float *)ring;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int s = c + 4 * h + e;  // slot at phase 0 (<= 30)
-      woff[e] = (uint32_t)((g16f_prow(4 * h + e) * kG16FRowStride + s) * sizeof(float));
-      if (s >= kG16FSlots - 32) wwrap |= 1u << e;
+      woff[e] = rb + (uint32_t)((g16f_prow(4 * h + e) * kG16FRowStride + s) * sizeof(float));
+      woff2[e] = woff[e] + 2 * kChunk * sizeof(float) -
+                 (s >= kG16FSlots - 32 ? kG16FSlots * sizeof(float) : 0);
     }
   }
   __device__ __forceinline__ void fetchA(const PairGeom<G> &geo, int band) {
@@ -191,15 +193,17 @@ struct G16FProducer {
     cC.advance(geo);
     if (more && cC.jv0 == 0) fetchA(geo, cC.band);
   }
-  // acc -> ring slots of chunk phase ph (chunk index mod 3)
-  __device__ __forceinline__ void store(char *ring, int ph) const {
+  // acc -> ring slots of chunk phase ph (chunk index mod 3): per element one address, the four
+  // pairs' slots at immediate offsets from it
+  __device__ __forceinline__ void store(int ph) const {
+    typedef __attribute__((address_space(3))) float lds_float;
+    const uint32_t po = (uint32_t)(ph * kChunk * sizeof(float));
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      uint32_t a = woff[e] + (uint32_t)(ph * kChunk * sizeof(float));
-      if (ph == 2 && ((wwrap >> e) & 1)) a -= (uint32_t)(kG16FSlots * sizeof(float));
+      const uint32_t a = ph == 2 ? woff2[e] : woff[e] + po;
 #pragma unroll
       for (int g = 0; g < 4; ++g)
-        *reinterpret_cast<float *>(ring + a + g * 16 * kG16FRowStride * sizeof(float)) = acc[g][e];
+        *(lds_float *)(uintptr_t)(a + g * 16 * kG16FRowStride * sizeof(float)) = acc[g][e];
     }
   }
 };
@@ -228,18 +232,17 @@ template <int KB, int NC>
 __device__ void g16f_produce(G16FProducer<KB, NC> &P, const PairGeom<16> &geo, float *ring,
                              char *stage) {
   constexpr size_t kSlot = g16f_stage_bytes<KB>() / 2;
-  char *rbytes = reinterpret_cast<char *>(ring);
   const int lane = threadIdx.x & 63;
   const int nchunks = geo.columns() / kChunk;
   const int nperiods = nchunks + 1;
   for (int i = lane; i < kWave * kG16FRowStride; i += 64) ring[i] = 0.0f;
-  P.init_writer();
+  P.init_writer(ring);
   P.cB.start(geo, 0);
   P.cC.start(geo, 0);
   P.fetchA(geo, 0);
   P.fetchB_all(geo);                     // chunk 0
   P.compute(geo, nchunks > 1);
-  P.store(rbytes, 0);
+  P.store(0);
   if (nchunks > 1) {
     P.fetchB_all(geo);                   // chunk 1
     P.compute(geo, nchunks > 2);
@@ -250,12 +253,24 @@ __device__ void g16f_produce(G16FProducer<KB, NC> &P, const PairGeom<16> &geo, f
   }
   if (nchunks > 3) P.fetchB_share(geo);  // chunk 3: staged in window 0
   int ph = 1;  // phase (index mod 3) of the chunk written next
-  for (int i = 0; i <= nperiods; ++i) {
+  int i = 0;
+  // windows in which every step applies (the loop below, without its tests)
+  for (; i + 4 < nchunks; ++i) {
+    __syncthreads();  // beta_(i-1)
+    const int k = i + 1;
+    P.stage(stage + ((k + 2) & 1) * kSlot);
+    P.fetchB_share(geo);
+    P.store(ph);
+    P.unstage(stage + ((k + 1) & 1) * kSlot);
+    P.compute(geo, true);
+    ph = ph == 2 ? 0 : ph + 1;
+  }
+  for (; i <= nperiods; ++i) {
     __syncthreads();  // beta_(i-1)
     const int k = i + 1;  // acc holds chunk k
     if (k + 2 < nchunks) P.stage(stage + ((k + 2) & 1) * kSlot);  // chunk i + 3
     if (k + 3 < nchunks) P.fetchB_share(geo);                       // chunk i + 4
-    if (k < nchunks) P.store(rbytes, ph);
+    if (k < nchunks) P.store(ph);
     if (k + 1 < nchunks) {                                          // chunk i + 2
       P.unstage(stage + ((k + 1) & 1) * kSlot);
       P.compute(geo, k + 2 < nchunks);
