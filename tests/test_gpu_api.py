@@ -223,7 +223,7 @@ def test_golden_fixtures_through_the_api():
     g = _golden("gram_f32_L100_d8_o2.npz")
     X, Y = cuda(g["X"]), cuda(g["Y"])
     for prec, bar in (("ref", 5e-5), ("fp32", 1e-4), ("fp16x2", 1e-5), ("bf16x3", 1e-5),
-                      ("fp64", 2e-7)):
+                      ("fp64", 2e-7), ("fp16x2f", 1e-5)):
         got = sig.SigPDE(lin, 2, precision=prec).gram(X, Y).double().cpu().numpy()
         assert rel_fro(got, g["K64"]) < bar, prec
     g = _golden("robust_f64_L64_d16_o1.npz")
@@ -234,7 +234,7 @@ def test_golden_fixtures_through_the_api():
     assert rel_fro(K, g["K"]) < 1e-9
 
 
-@pytest.mark.parametrize("prec", ["ref", "fp32", "fp16x2", "bf16x3", "fp64"])
+@pytest.mark.parametrize("prec", ["ref", "fp32", "fp16x2", "bf16x3", "fp64", "fp16x2f"])
 def test_empty_batches_every_precision(prec):
     """A batch of no paths gives an empty result of the right shape on every method, precision
     and dtype (no launch with an empty grid)."""
@@ -255,7 +255,7 @@ def test_empty_batches_every_precision(prec):
 
 
 @pytest.mark.parametrize("prec,bar", [("ref", 1e-10), ("fp32", 1e-5), ("fp16x2", 1e-6),
-                                      ("bf16x3", 1e-6), ("fp64", 1e-10)])
+                                      ("bf16x3", 1e-6), ("fp64", 1e-10), ("fp16x2f", 1e-5)])
 def test_single_increment_paths(prec, bar):
     """Paths of length 2 (one increment each), with lengths unequal to the other side's, at orders
     0 and 2, against the float64 oracle (float64 inputs; float32 output rounding on the fused
