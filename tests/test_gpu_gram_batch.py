@@ -46,7 +46,7 @@ def test_batch_equals_separate_grams(precision, dtype, order):
         assert torch.equal(blk, ref), (precision, bx, by)
 
 
-@pytest.mark.parametrize("precision", ["fp16x2", "fp32"])
+@pytest.mark.parametrize("precision", ["fp16x2", "fp32", "fp16x2f"])
 def test_batch_symmetric_and_scaled(precision):
     L, d, order = 33, 8, 1
     xs = [paths(b, L, d, 70 + b) for b in (6, 11, 1)]
@@ -70,7 +70,7 @@ def test_batch_symmetric_and_scaled(precision):
         # against the float64 oracle on the same float32 inputs
         k64 = O.linear_gram(x.double().cpu().numpy(), x.double().cpu().numpy(), order,
                             sx=sb.double().cpu().numpy(), sy=sb.double().cpu().numpy())
-        assert rel_fro(blk.double().cpu().numpy(), k64) < (1e-5 if precision == "fp16x2" else 1e-4)
+        assert rel_fro(blk.double().cpu().numpy(), k64) < (1e-4 if precision == "fp32" else 1e-5)
         off += b * b
         x0 += b
 
